@@ -1,0 +1,490 @@
+// The C ABI (include/asph.h, asph_debug.h, asph_batch.h) over the engine.
+// No exception crosses this boundary: each entry point returns a status and
+// records the message for asph_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/asph.h"
+#include "../../include/asph_batch.h"
+#include "../../include/asph_debug.h"
+#include "engine.hpp"
+
+namespace asph
+{
+int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
+                     uint32_t* count);
+}
+
+struct asph_simulation
+{
+    std::unique_ptr<asph::Engine> engine;
+    uint32_t scene = 0;
+    bool adaptive = false;
+    float lastDt = -1.0F;
+};
+
+struct asph_batch
+{
+    std::unique_ptr<asph::Engine> engine;
+};
+
+namespace
+{
+
+thread_local std::string g_error;
+
+template <class F>
+int guard(F&& f)
+{
+    try
+    {
+        f();
+        g_error.clear();
+        return ASPH_OK;
+    }
+    catch (const asph::Error& e)
+    {
+        g_error = e.what();
+        return e.code;
+    }
+    catch (const std::bad_alloc&)
+    {
+        g_error = "host allocation failed";
+        return ASPH_ERR_CAPACITY;
+    }
+    catch (const std::exception& e)
+    {
+        g_error = e.what();
+        return ASPH_ERR_INVALID;
+    }
+}
+
+void need(const void* p, const char* what)
+{
+    if (p == nullptr)
+    {
+        throw asph::Error(ASPH_ERR_INVALID, std::string(what) + " is NULL");
+    }
+}
+
+asph::Engine& engineOf(asph_simulation* s)
+{
+    need(s, "sim");
+    return *s->engine;
+}
+
+asph::Engine& engineOf(asph_batch* b)
+{
+    need(b, "batch");
+    return *b->engine;
+}
+
+void checkScene(asph::Engine& e, uint32_t id)
+{
+    if (!e.sceneActive(id))
+    {
+        throw asph::Error(ASPH_ERR_INVALID, "no such scene");
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* asph_last_error(void) { return g_error.c_str(); }
+
+int asph_default_parameters(asph_parameters* p)
+{
+    return guard([&] {
+        need(p, "p");
+        std::memset(p, 0, sizeof(*p));
+        p->domain.min = {-1.F, -1.F, -1.F};
+        p->domain.max = {1.F, 1.F, 1.F};
+        p->domain.friction = 0.01F;
+        p->threads_per_block = 256;
+        p->tait_gamma = 5.0F;
+    });
+}
+
+int asph_default_refinement_parameters(asph_refinement_parameters* r)
+{
+    return guard([&] {
+        need(r, "r");
+        std::memset(r, 0, sizeof(*r));
+        r->enabled = 0;
+        r->min_mass_ratio = 0.3F;
+        r->max_mass_ratio = 8.0F;
+        r->max_particle_count = 1000000;
+        r->max_batch_ratio = 0.2F;
+        r->initial_cooldown = 10;
+        r->cooldown = 10;
+        r->criterion_type = ASPH_CRITERION_VELOCITY;
+        r->splitting = {0.65F, 0.70F, 0.077F, 0.077F};
+        r->velocity = {2.0F, 0.1F};
+        r->vorticity = {1.0F, 0.3F};
+        r->curvature = {2.0F, 0.6F};
+        r->interface_ratio = {0.05F, 0.15F};
+    });
+}
+
+int asph_boundary_generate(const asph_domain* domain, float particle_spacing, float rest_density,
+                           float smoothing_radius, asph_boundary_particle* out, uint32_t* count)
+{
+    return guard([&] {
+        need(domain, "domain");
+        need(count, "count");
+        if (!(particle_spacing > 0.0F) || !(smoothing_radius > 0.0F))
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "spacing and smoothing radius must be positive");
+        }
+        const int rc = asph::boundaryGenerate(*domain, particle_spacing, rest_density, smoothing_radius, out, count);
+        if (rc != ASPH_OK)
+        {
+            throw asph::Error(rc, "output array too small");
+        }
+    });
+}
+
+int asph_create(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
+                const asph_boundary_particle* boundary, uint32_t boundary_count,
+                const asph_refinement_parameters* refinement, uint32_t max_fluid_capacity,
+                uint32_t max_boundary_capacity, asph_simulation** out)
+{
+    return guard([&] {
+        need(params, "params");
+        need(out, "out");
+        *out = nullptr;
+        const bool adaptive = refinement != nullptr && refinement->enabled != 0U;
+        const uint32_t cap = adaptive ? refinement->max_particle_count : max_fluid_capacity;
+        if (boundary_count > max_boundary_capacity)
+        {
+            throw asph::Error(ASPH_ERR_CAPACITY, "boundary count exceeds the boundary capacity");
+        }
+        const asph::GridGeom g = asph::makeGrid(params->domain, params->base_smoothing_radius);
+        asph::EngineConfig cfg;
+        cfg.particleCapacity = cap;
+        cfg.boundaryCapacity = max_boundary_capacity;
+        cfg.cellCapacity = g.cells() + 1U;
+        cfg.sceneSlots = 1;
+        cfg.adaptive = adaptive;
+        cfg.useGraph = false;
+        auto sim = std::make_unique<asph_simulation>();
+        sim->engine = std::make_unique<asph::Engine>(cfg);
+        sim->engine->setDebugCapture(adaptive);
+        sim->scene = sim->engine->addScene(*params, positions, count, boundary, boundary_count, refinement, cap,
+                                           0.0F);
+        sim->adaptive = adaptive;
+        *out = sim.release();
+    });
+}
+
+int asph_destroy(asph_simulation* sim)
+{
+    return guard([&] { delete sim; });
+}
+
+int asph_update(asph_simulation* sim, float dt)
+{
+    return guard([&] {
+        asph::Engine& e = engineOf(sim);
+        if (!std::isfinite(dt))
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "dt must be finite");
+        }
+        if (dt != sim->lastDt)
+        {
+            e.setDt(sim->scene, dt);
+            sim->lastDt = dt;
+        }
+        e.step(1);
+        e.synchronize();  // update is synchronous (SphSimulation.cu:119)
+    });
+}
+
+int asph_update_domain(asph_simulation* sim, const asph_domain* domain, const asph_boundary_particle* boundary,
+                       uint32_t boundary_count)
+{
+    return guard([&] {
+        need(domain, "domain");
+        if (boundary_count > 0)
+        {
+            need(boundary, "boundary");
+        }
+        engineOf(sim).updateDomain(sim->scene, *domain, boundary, boundary_count);
+    });
+}
+
+int asph_fluid_count(asph_simulation* sim, uint32_t* count)
+{
+    return guard([&] {
+        need(count, "count");
+        *count = engineOf(sim).count(sim->scene);
+    });
+}
+
+int asph_boundary_count(asph_simulation* sim, uint32_t* count)
+{
+    return guard([&] {
+        need(count, "count");
+        *count = engineOf(sim).boundaryCount(sim->scene);
+    });
+}
+
+int asph_average_neighbor_count(asph_simulation* sim, float* out)
+{
+    return guard([&] {
+        need(out, "out");
+        *out = engineOf(sim).averageNeighborCount(sim->scene);
+    });
+}
+
+int asph_get_density_info(asph_simulation* sim, float threshold, asph_density_info* out)
+{
+    return guard([&] {
+        need(out, "out");
+        *out = engineOf(sim).densityInfo(sim->scene, threshold);
+    });
+}
+
+int asph_enable_adaptive_refinement(asph_simulation* sim)
+{
+    return guard([&] { engineOf(sim).enableRefinement(sim->scene); });
+}
+
+int asph_get_fluid_view(asph_simulation* sim, asph_fluid_view* out)
+{
+    return guard([&] {
+        need(out, "out");
+        *out = engineOf(sim).view(sim->scene);
+    });
+}
+
+int asph_copy_fluid(asph_simulation* sim, uint32_t mask, asph_vec4* positions, asph_vec4* velocities,
+                    asph_vec4* accelerations, float* densities, float* radii, float* smoothing_radii, float* masses)
+{
+    return guard([&] {
+        engineOf(sim).copyFluid(sim->scene, mask, positions, velocities, accelerations, densities, radii,
+                                smoothing_radii, masses);
+    });
+}
+
+// ------------------------------------------------------------------ debug --
+
+int asph_debug_set_fluid(asph_simulation* sim, uint32_t count, const asph_vec4* positions,
+                         const asph_vec4* velocities, const asph_vec4* accelerations, const float* densities,
+                         const float* radii, const float* smoothing_radii, const float* masses)
+{
+    return guard([&] {
+        engineOf(sim).setFluid(sim->scene, count, positions, velocities, accelerations, densities, radii,
+                               smoothing_radii, masses);
+    });
+}
+
+int asph_debug_set_frame(asph_simulation* sim, uint32_t frame)
+{
+    return guard([&] { engineOf(sim).setFrame(sim->scene, frame); });
+}
+
+int asph_debug_grid_dims(asph_simulation* sim, int32_t dims[3])
+{
+    return guard([&] {
+        need(dims, "dims");
+        const asph::GridGeom g = engineOf(sim).gridGeom(sim->scene);
+        dims[0] = g.gx;
+        dims[1] = g.gy;
+        dims[2] = g.gz;
+    });
+}
+
+int asph_debug_grid(asph_simulation* sim, int32_t* sorted_keys, int32_t* sorted_indices, int32_t* cell_start,
+                    int32_t* cell_end)
+{
+    return guard([&] { engineOf(sim).grid(sim->scene, sorted_keys, sorted_indices, cell_start, cell_end); });
+}
+
+int asph_debug_neighbor_sets(asph_simulation* sim, uint32_t* counts, uint64_t* fingerprints)
+{
+    return guard([&] { engineOf(sim).neighborSets(sim->scene, counts, fingerprints); });
+}
+
+int asph_debug_refinement(asph_simulation* sim, asph_refinement_trace* trace, uint32_t* merge_list,
+                          uint32_t* merge_candidates, uint32_t* pair_keep_ids, uint32_t* split_parents)
+{
+    return guard([&] {
+        engineOf(sim).refinementTrace(sim->scene, trace, merge_list, merge_candidates, pair_keep_ids, split_parents);
+    });
+}
+
+// ------------------------------------------------------------------ batch --
+
+int asph_batch_create(const asph_batch_config* config, asph_batch** out)
+{
+    return guard([&] {
+        need(config, "config");
+        need(out, "out");
+        *out = nullptr;
+        asph::EngineConfig cfg;
+        cfg.particleCapacity = config->max_particles;
+        cfg.boundaryCapacity = config->max_boundary;
+        cfg.cellCapacity = config->max_cells;
+        cfg.sceneSlots = config->max_scenes == 0 ? 1U : config->max_scenes;
+        cfg.listCap = config->list_capacity == 0 ? 128U : config->list_capacity;
+        cfg.adaptive = (config->flags & ASPH_BATCH_ADAPTIVE) != 0U;
+        cfg.useGraph = (config->flags & ASPH_BATCH_GRAPH) != 0U;
+        auto b = std::make_unique<asph_batch>();
+        b->engine = std::make_unique<asph::Engine>(cfg);
+        *out = b.release();
+    });
+}
+
+int asph_batch_destroy(asph_batch* batch)
+{
+    return guard([&] { delete batch; });
+}
+
+int asph_batch_add_scene(asph_batch* batch, const asph_parameters* params, const asph_vec4* positions,
+                         uint32_t count, const asph_boundary_particle* boundary, uint32_t boundary_count,
+                         const asph_refinement_parameters* refinement, uint32_t capacity, float dt,
+                         uint32_t* scene_id)
+{
+    return guard([&] {
+        need(params, "params");
+        need(scene_id, "scene_id");
+        if (!(dt > 0.0F) || !std::isfinite(dt))
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "dt must be positive and finite");
+        }
+        *scene_id = engineOf(batch).addScene(*params, positions, count, boundary, boundary_count, refinement,
+                                             capacity, dt);
+    });
+}
+
+int asph_batch_remove_scene(asph_batch* batch, uint32_t scene_id)
+{
+    return guard([&] { engineOf(batch).removeScene(scene_id); });
+}
+
+int asph_batch_step(asph_batch* batch, uint32_t steps)
+{
+    return guard([&] { engineOf(batch).step(steps); });
+}
+
+int asph_batch_synchronize(asph_batch* batch)
+{
+    return guard([&] { engineOf(batch).synchronize(); });
+}
+
+int asph_batch_step_timed(asph_batch* batch, float* milliseconds)
+{
+    return guard([&] {
+        need(milliseconds, "milliseconds");
+        *milliseconds = engineOf(batch).timedStep();
+    });
+}
+
+int asph_batch_scene_count(asph_batch* batch, uint32_t scene_id, uint32_t* count)
+{
+    return guard([&] {
+        need(count, "count");
+        asph::Engine& e = engineOf(batch);
+        checkScene(e, scene_id);
+        *count = e.count(scene_id);
+    });
+}
+
+int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, asph_vec4* positions,
+                          asph_vec4* velocities, asph_vec4* accelerations, float* densities, float* radii,
+                          float* smoothing_radii, float* masses)
+{
+    return guard([&] {
+        asph::Engine& e = engineOf(batch);
+        checkScene(e, scene_id);
+        e.copyFluid(scene_id, mask, positions, velocities, accelerations, densities, radii, smoothing_radii, masses);
+    });
+}
+
+int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t radix_bits, uint32_t use_graph)
+{
+    return guard([&] {
+        asph::Engine& e = engineOf(batch);
+        if (block_stream < 64 || block_stream > 1024 || (block_stream % 64) != 0)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "block_stream must be a multiple of 64 in [64, 1024]");
+        }
+        if (radix_bits != 4 && radix_bits != 6 && radix_bits != 7 && radix_bits != 8)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "radix_bits must be 4, 6, 7 or 8");
+        }
+        asph::LaunchCfg lc = e.config().launch;
+        lc.blockStream = block_stream;
+        lc.radixBits = static_cast<int>(radix_bits);
+        e.setLaunch(lc);
+        e.setUseGraph(use_graph != 0U);
+    });
+}
+
+int asph_batch_memory(asph_batch* batch, uint64_t* allocated, uint64_t* particles_used)
+{
+    return guard([&] {
+        asph::Engine& e = engineOf(batch);
+        if (allocated != nullptr)
+        {
+            *allocated = e.bytesAllocated();
+        }
+        if (particles_used != nullptr)
+        {
+            *particles_used = e.particlesUsed();
+        }
+    });
+}
+
+int asph_batch_profile_step(asph_batch* batch, float* ms, uint32_t capacity, uint32_t* count)
+{
+    return guard([&] {
+        need(count, "count");
+        const std::vector<float> t = engineOf(batch).profileStep();
+        *count = static_cast<uint32_t>(t.size());
+        for (uint32_t i = 0; i < t.size() && i < capacity && ms != nullptr; ++i)
+        {
+            ms[i] = t[i];
+        }
+    });
+}
+
+const char* asph_profile_stage_name(uint32_t stage) { return asph::stageName(stage); }
+
+uint64_t asph_scene_bytes(uint32_t capacity, uint32_t boundary_count, uint32_t cells, uint32_t adaptive,
+                          uint32_t list_capacity)
+{
+    return asph::sceneBytes(capacity, boundary_count, cells, adaptive != 0U,
+                            list_capacity == 0 ? 128U : list_capacity);
+}
+
+int asph_grid_cells(const asph_domain* domain, float smoothing_radius, uint32_t* cells)
+{
+    return guard([&] {
+        need(domain, "domain");
+        need(cells, "cells");
+        *cells = asph::makeGrid(*domain, smoothing_radius).cells();
+    });
+}
+
+int asph_device_info(char* buffer, uint32_t length)
+{
+    return guard([&] {
+        need(buffer, "buffer");
+        int dev = 0;
+        asph::hipCheck(hipGetDevice(&dev), "hipGetDevice");
+        hipDeviceProp_t prop {};
+        asph::hipCheck(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+        std::snprintf(buffer, length, "%s (%s), %d CUs, %.1f GiB", prop.name, prop.gcnArchName,
+                      prop.multiProcessorCount, static_cast<double>(prop.totalGlobalMem) / (1024.0 * 1024.0 * 1024.0));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,182 @@
+// Host side of the HIP engine: device pools, the scene table, and the
+// per-step launch sequence (optionally captured once as a hipGraph and
+// replayed). A Simulation of the reference interface is an Engine holding one
+// scene; a batch is an Engine holding many.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/asph.h"
+#include "../../include/asph_debug.h"
+#include "kernels.hpp"
+
+namespace asph
+{
+
+struct Error : std::runtime_error
+{
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what);
+
+// First-fit allocator of index ranges inside a fixed pool.
+class RangeAllocator
+{
+public:
+    explicit RangeAllocator(uint64_t capacity = 0);
+    // Returns the offset of a free range of n elements, or UINT64_MAX.
+    uint64_t alloc(uint64_t n);
+    void release(uint64_t off, uint64_t n);
+    uint64_t capacity() const { return cap_; }
+    uint64_t used() const { return used_; }
+
+private:
+    struct Range
+    {
+        uint64_t off, n;
+    };
+    std::vector<Range> free_;
+    uint64_t cap_ = 0;
+    uint64_t used_ = 0;
+};
+
+// Grid geometry of NeighborGrid::createGrid (NeighborGrid.cu:108-121).
+struct GridGeom
+{
+    int gx = 0, gy = 0, gz = 0;
+    float cs[3] = {0.F, 0.F, 0.F};
+    uint32_t cells() const { return static_cast<uint32_t>(gx) * static_cast<uint32_t>(gy) * static_cast<uint32_t>(gz); }
+};
+GridGeom makeGrid(const asph_domain& d, float h0);
+
+struct EngineConfig
+{
+    uint32_t particleCapacity = 0;  // pool of fluid particle slots
+    uint32_t boundaryCapacity = 0;  // pool of boundary particle slots
+    uint32_t cellCapacity = 0;      // pool of grid cells
+    uint32_t sceneSlots = 1;        // scene table size
+    uint32_t listCap = 128;         // neighbour-list entries per particle
+    bool adaptive = false;          // allocate refinement scratch
+    bool useGraph = false;          // capture and replay one step as a hipGraph
+    LaunchCfg launch;
+};
+
+// Bytes of device memory a scene of this shape needs inside an engine
+// (used for admission against a memory budget).
+const char* stageName(uint32_t i);
+
+uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap);
+
+class Engine
+{
+public:
+    explicit Engine(const EngineConfig& cfg);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Adds a scene; returns its slot id. capacity is the fluid capacity.
+    uint32_t addScene(const asph_parameters& p, const asph_vec4* positions, uint32_t count,
+                      const asph_boundary_particle* boundary, uint32_t nb, const asph_refinement_parameters* r,
+                      uint32_t capacity, float dt);
+    void removeScene(uint32_t id);
+    bool sceneActive(uint32_t id) const;
+    void setDt(uint32_t id, float dt);
+    void updateDomain(uint32_t id, const asph_domain& d, const asph_boundary_particle* boundary, uint32_t nb);
+    void enableRefinement(uint32_t id);
+    void setFrame(uint32_t id, uint32_t frame);
+
+    // Enqueue `steps` steps of every active scene (asynchronous).
+    void step(uint32_t steps);
+    void synchronize();
+
+    uint32_t count(uint32_t id);  // synchronous
+    uint32_t boundaryCount(uint32_t id) const;
+    uint32_t capacity(uint32_t id) const;
+    void copyFluid(uint32_t id, uint32_t mask, asph_vec4* pos, asph_vec4* vel, asph_vec4* acc, float* rho,
+                   float* rad, float* h, float* m);
+    void setFluid(uint32_t id, uint32_t count, const asph_vec4* pos, const asph_vec4* vel, const asph_vec4* acc,
+                  const float* rho, const float* rad, const float* h, const float* m);
+    asph_fluid_view view(uint32_t id);
+    float averageNeighborCount(uint32_t id);
+    asph_density_info densityInfo(uint32_t id, float threshold);
+    void neighborSets(uint32_t id, uint32_t* counts, uint64_t* prints);
+    void grid(uint32_t id, int32_t* keys, int32_t* idx, int32_t* cellStart, int32_t* cellEnd);
+    GridGeom gridGeom(uint32_t id) const;
+    void refinementTrace(uint32_t id, asph_refinement_trace* t, uint32_t* mergeList, uint32_t* cand,
+                         uint32_t* pairKeep, uint32_t* splitParents);
+    void setDebugCapture(bool on) { debugCapture_ = on; }
+
+    uint64_t bytesAllocated() const { return bytes_; }
+    uint64_t particlesUsed() const { return pAlloc_.used(); }
+    uint32_t activeScenes() const;
+    const EngineConfig& config() const { return cfg_; }
+    hipStream_t stream() const { return stream_; }
+    void setLaunch(const LaunchCfg& lc);
+    // One eager step with an event between stages (stage times in ms).
+    std::vector<float> profileStep();
+    // One step bracketed by events (ms).
+    float timedStep();
+    void setUseGraph(bool on);
+
+private:
+    struct HostScene
+    {
+        bool active = false;
+        SceneDev dev {};
+        uint32_t cellRegion = 0;  // nCells + 1
+        bool adaptive = false;
+        uint32_t frame = 0;  // host mirror of the device frame counter
+        uint32_t count0 = 0;
+        std::vector<F4> boundarySorted;
+        std::vector<int32_t> boundaryKeysSorted;
+    };
+
+    template <class T>
+    T* alloc(size_t n);
+    void buildCtx();
+    void uploadScene(uint32_t id);
+    void uploadOrder();
+    void uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32_t nb);
+    void enqueueStep(bool refine, std::vector<hipEvent_t>* marks);
+    bool anyRefineNext() const;
+    void advanceFrames();
+    void invalidateGraphs();
+    SceneState readState(uint32_t id);
+
+    EngineConfig cfg_;
+    hipStream_t stream_ = nullptr;
+    std::vector<void*> allocations_;
+    uint64_t bytes_ = 0;
+    Ctx ctx_ {};
+    int keyBits_ = 1;
+    int sortParity_ = 0;
+    int csortParity_ = 0;
+    int sceneBits_ = 1;
+    RangeAllocator pAlloc_, bAlloc_, cAlloc_;
+    std::vector<HostScene> scenes_;
+    // device scratch not in Ctx
+    uint32_t* hist_ = nullptr;
+    uint32_t* digitTotals_ = nullptr;
+    uint32_t* tileSums_ = nullptr;
+    uint32_t* scanTotal_ = nullptr;
+    uint32_t* nOrderDev_ = nullptr;
+    uint32_t* cellOrderDev_ = nullptr;
+    double* infoD_ = nullptr;
+    uint32_t* infoU_ = nullptr;
+    uint32_t* diagTotal_ = nullptr;
+    bool debugCapture_ = false;
+    hipGraphExec_t graphPlain_ = nullptr;
+    hipGraphExec_t graphRefine_ = nullptr;
+    hipGraph_t graphPlainG_ = nullptr;
+    hipGraph_t graphRefineG_ = nullptr;
+};
+
+}  // namespace asph

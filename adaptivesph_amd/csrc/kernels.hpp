@@ -1,0 +1,138 @@
+// Launchers of the HIP kernels (sort.hip, sph.hip). All launches are
+// asynchronous on the given stream and read element counts from device memory
+// where noted, so one step is a static sequence of launches (graph-capturable).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "device_common.hpp"
+
+namespace asph
+{
+
+// ------------------------------------------------------------ sort.hip --
+size_t radixHistWords(uint32_t nMax, int radixBits);
+size_t scanTileWords(uint32_t nMax);
+// Returns 0 when the result is in (keysA, valsA), 1 when in (keysB, valsB).
+int radixSortU32(uint32_t* keysA, uint32_t* valsA, uint32_t* keysB, uint32_t* valsB, const uint32_t* nPtr,
+                 uint32_t nMax, int bits, int radixBits, uint32_t* hist, uint32_t* digitTotals, hipStream_t stream);
+int radixSortU64(uint64_t* keysA, uint32_t* valsA, uint64_t* keysB, uint32_t* valsB, const uint32_t* nPtr,
+                 uint32_t nMax, int bits, uint32_t* hist, uint32_t* digitTotals, hipStream_t stream);
+// out[i] = sum in[0, i) for i < n (n = *nPtr if nPtr else nMax); *total = sum.
+void exclusiveScanU32(uint32_t* in, uint32_t* out, const uint32_t* nPtr, uint32_t nMax, uint32_t* tileSums,
+                      uint32_t* total, bool zeroInput, hipStream_t stream);
+
+// ------------------------------------------------------------- sph.hip --
+
+// Device pointers of the engine's pools.
+struct Ctx
+{
+    // scene tables
+    const SceneDev* scenes;
+    SceneState* states;
+    uint32_t sceneSlots;        // number of scene slots (table size)
+    uint32_t* denseStarts;      // [sceneSlots] exclusive prefix of counts
+    uint32_t* sortedStarts;     // [sceneSlots] first sorted slot, in cell-base order
+    const uint32_t* cellOrder;  // [sceneSlots] scene ids in cell-base order
+    const uint32_t* nOrderPtr;  // device: active scenes in cellOrder
+    uint32_t* nTotal;           // device: active particles this step
+
+    // canonical particle arrays (interface layout)
+    F4* pos;
+    F4* vel;
+    F4* acc;
+    float* rho;
+    float* rad;
+    float* h;
+    float* m;
+    uint32_t pCap;  // pool capacity (particles)
+
+    // neighbour grid
+    uint32_t* keysA;
+    uint32_t* valsA;
+    uint32_t* keysB;
+    uint32_t* valsB;
+    uint32_t* sortedKeys;  // result buffers of the sort (set per step)
+    uint32_t* sortedVals;
+    uint32_t* cellCount;  // [cCap]
+    uint32_t* cellOff;    // [cCap + 1]
+    uint32_t cCap;
+
+    // sorted shadow copies
+    F4* s0;  // (x, y, z, (2h)^2)
+    F4* s1;  // (vx, vy, vz, h)
+    float* sm;
+    F4* s2;  // (rho, p, p / rho^2, 0)
+    uint32_t* sscene;
+    uint32_t* ncount;
+    uint32_t* list;  // [listCap * pCap], entry-major
+    uint32_t listCap;
+
+    // boundary (sorted by cell per scene)
+    F4* bs0;  // (x, y, z, psi)
+    float* bvisc;
+    uint32_t* bcellOff;  // [cCap]: absolute boundary slot of each cell start
+
+    // refinement scratch (dense-indexed)
+    uint64_t* ckeysA;
+    uint32_t* cvalsA;
+    uint64_t* ckeysB;
+    uint32_t* cvalsB;
+    uint64_t* csortedKeys;
+    uint32_t* csortedVals;
+    uint32_t* cand;
+    uint32_t* removal;
+    uint32_t* keep;
+    uint32_t* rprefix;
+    F4* xpos;
+    F4* xvel;
+    F4* xacc;
+    float* xrho;
+    float* xrad;
+    float* xh;
+    float* xm;
+    uint32_t* mergeListDbg;  // optional capture of the merge list (sorted vals)
+};
+
+struct LaunchCfg
+{
+    uint32_t blockStream = 256;  // streaming kernels
+    uint32_t blockNeighbor = 256;  // neighbour kernels
+    int radixBits = 8;
+};
+
+// Step stages, in call order.
+void launchPrep(const Ctx& c, hipStream_t s);
+void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchSceneRanges(const Ctx& c, hipStream_t s);
+void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+
+// Refinement stages (AdaptiveSphSimulation::performAdaptiveRefinement).
+void launchMergeCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchMergeSelect(const Ctx& c, hipStream_t s);
+void launchMergeCandidates(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchResolvePairs(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchMergeApply(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchCompact(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchAfterMerge(const Ctx& c, hipStream_t s);
+void launchCopyBack(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchSplitCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchSplitSelect(const Ctx& c, hipStream_t s);
+void launchSplitApply(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchKickRefined(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchCopyMergeList(const Ctx& c, hipStream_t s);
+
+// Diagnostics of one scene (calculateAverageNeighborCount, getDensityInfo,
+// neighbour-set fingerprints).
+void launchNeighborSets(const Ctx& c, uint32_t scene, uint32_t count, uint32_t* counts, uint64_t* prints,
+                        uint32_t* total, hipStream_t s);
+void launchDensityInfo(const Ctx& c, uint32_t scene, uint32_t count, float threshold, double* partials,
+                       uint32_t* upartials, hipStream_t s);
+constexpr uint32_t kInfoBlocks = 256;
+
+}  // namespace asph
