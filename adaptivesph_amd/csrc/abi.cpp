@@ -159,8 +159,13 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
         need(params, "params");
         need(out, "out");
         *out = nullptr;
+        asph::validateParams(*params);
         const bool adaptive = refinement != nullptr && refinement->enabled != 0U;
         const uint32_t cap = adaptive ? refinement->max_particle_count : max_fluid_capacity;
+        if (count > cap)
+        {
+            throw asph::Error(ASPH_ERR_CAPACITY, "particle count exceeds the fluid capacity");
+        }
         if (boundary_count > max_boundary_capacity)
         {
             throw asph::Error(ASPH_ERR_CAPACITY, "boundary count exceeds the boundary capacity");

@@ -17,11 +17,13 @@ namespace asph
 
 constexpr uint32_t kNoCandidate = 0xFFFFFFFFU;  // refinement/Common.cuh:119-122
 constexpr uint32_t kWave = 64;
+constexpr uint32_t kGroup = 8;  // sorted slots per culling box
 
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;
 constexpr uint32_t kSceneAdaptive = 2U;
 constexpr uint32_t kSceneFixCellStart = 4U;
+constexpr uint32_t kSceneBoundaryInside = 8U;  // some boundary particle lies in an interior cell
 
 // Read-only per-scene description (uploaded when the scene set changes).
 struct SceneDev

@@ -51,6 +51,19 @@ uint64_t RangeAllocator::alloc(uint64_t n)
     return UINT64_MAX;
 }
 
+void RangeAllocator::grow(uint64_t capacity)
+{
+    if (capacity <= cap_)
+    {
+        return;
+    }
+    const uint64_t extra = capacity - cap_;
+    const uint64_t off = cap_;
+    cap_ = capacity;
+    used_ += extra;  // release() subtracts it again
+    release(off, extra);
+}
+
 void RangeAllocator::release(uint64_t off, uint64_t n)
 {
     if (n == 0)
@@ -126,6 +139,8 @@ int bitsFor(uint64_t maxValue)
     return b;
 }
 
+}  // namespace
+
 void validateParams(const asph_parameters& p)
 {
     auto bad = [](const char* m) { throw Error(ASPH_ERR_INVALID, m); };
@@ -150,9 +165,15 @@ void validateParams(const asph_parameters& p)
     {
         bad("threads_per_block must be positive");
     }
+    const float cellWidth = (2.0F * p.base_smoothing_radius) * 2.0F;
+    const double cells = std::ceil(static_cast<double>(p.domain.max.x - p.domain.min.x) / cellWidth) *
+                         std::ceil(static_cast<double>(p.domain.max.y - p.domain.min.y) / cellWidth) *
+                         std::ceil(static_cast<double>(p.domain.max.z - p.domain.min.z) / cellWidth);
+    if (!(cells >= 1.0) || cells > 1.0e9)
+    {
+        bad("neighbour grid would need more than 1e9 cells (smoothing radius too small for the domain)");
+    }
 }
-
-}  // namespace
 
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
@@ -219,8 +240,10 @@ Engine::Engine(const EngineConfig& cfg)
         c.cCap = C;
         c.s0 = alloc<F4>(P);
         c.s1 = alloc<F4>(P);
-        c.sm = alloc<float>(P);
+        c.shm = alloc<float2>(P);
         c.s2 = alloc<F4>(P);
+        c.gbox0 = alloc<F4>(P / kGroup + 1);
+        c.gbox1 = alloc<F4>(P / kGroup + 1);
         c.sscene = alloc<uint32_t>(P);
         c.ncount = alloc<uint32_t>(P);
         c.listCap = cfg_.listCap;
@@ -600,6 +623,23 @@ void Engine::uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32
         }
     }
     d.bLostEnd = lost;
+    // Boundary walks are skipped for queries whose cell box avoids the
+    // outermost cells, which is exact only if every boundary particle lies in
+    // an outermost cell (true for StaticBoundaryDomain walls outside the box).
+    bool inside = false;
+    for (uint32_t k = 0; k < nb; ++k)
+    {
+        const int32_t key = hs.boundaryKeysSorted[k];
+        const int cx = key % d.gx;
+        const int cy = (key / d.gx) % d.gy;
+        const int cz = key / (d.gx * d.gy);
+        if (cx != 0 && cy != 0 && cz != 0 && cx != d.gx - 1 && cy != d.gy - 1 && cz != d.gz - 1)
+        {
+            inside = true;
+            break;
+        }
+    }
+    d.flags = inside ? (d.flags | kSceneBoundaryInside) : (d.flags & ~kSceneBoundaryInside);
     d.bCount = nb;
     if (nb > 0)
     {
@@ -694,8 +734,21 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     const uint64_t cells = g.cells();
     cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
     bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
-    const uint64_t cBase = cAlloc_.alloc(cells + 1);
-    const uint64_t bOff = bAlloc_.alloc(nb);
+    // NeighborGrid::updateBoundarySize reallocates the cell arrays for the new
+    // domain (NeighborGrid.cu:51-85); the pools grow when the new grid or wall
+    // lattice does not fit.
+    uint64_t cBase = cAlloc_.alloc(cells + 1);
+    if (cBase == UINT64_MAX)
+    {
+        growCells(cAlloc_.capacity() + cells + 1);
+        cBase = cAlloc_.alloc(cells + 1);
+    }
+    uint64_t bOff = bAlloc_.alloc(nb);
+    if (bOff == UINT64_MAX)
+    {
+        growBoundary(bAlloc_.capacity() + nb);
+        bOff = bAlloc_.alloc(nb);
+    }
     if (cBase == UINT64_MAX || bOff == UINT64_MAX)
     {
         throw Error(ASPH_ERR_CAPACITY, "pool exhausted while updating the domain");
@@ -724,6 +777,56 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     uploadScene(id);
     uploadOrder();
     hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+template <class T>
+void Engine::regrow(T*& p, size_t oldN, size_t newN)
+{
+    T* q = nullptr;
+    hipCheck(hipMalloc(&q, std::max<size_t>(newN, 1) * sizeof(T)), "hipMalloc");
+    hipCheck(hipMemset(q, 0, std::max<size_t>(newN, 1) * sizeof(T)), "hipMemset");
+    if (oldN > 0)
+    {
+        hipCheck(hipMemcpy(q, p, oldN * sizeof(T), hipMemcpyDeviceToDevice), "hipMemcpy");
+    }
+    for (auto& a : allocations_)
+    {
+        if (a == p)
+        {
+            a = q;
+        }
+    }
+    (void)hipFree(p);
+    bytes_ += (newN - oldN) * sizeof(T);
+    p = q;
+}
+
+void Engine::growCells(uint64_t newCap)
+{
+    synchronize();
+    const size_t oldC = ctx_.cCap;
+    const size_t C = std::max<size_t>(newCap, oldC);
+    regrow(ctx_.cellCount, oldC, C);
+    regrow(ctx_.cellOff, oldC + 1, C + 1);
+    regrow(ctx_.bcellOff, oldC + 1, C + 1);
+    regrow(tileSums_, scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(oldC) + 1)),
+           scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(C) + 1)));
+    ctx_.cCap = static_cast<uint32_t>(C);
+    cAlloc_.grow(C);
+    keyBits_ = bitsFor(C - 1);
+    buildCtx();
+    invalidateGraphs();
+}
+
+void Engine::growBoundary(uint64_t newCap)
+{
+    synchronize();
+    const size_t oldB = std::max<uint64_t>(bAlloc_.capacity(), 1);
+    const size_t B = std::max<size_t>(newCap, oldB);
+    regrow(ctx_.bs0, oldB, B);
+    regrow(ctx_.bvisc, oldB, B);
+    bAlloc_.grow(B);
+    invalidateGraphs();
 }
 
 SceneState Engine::readState(uint32_t id)
@@ -905,7 +1008,21 @@ void Engine::step(uint32_t steps)
             if (exec == nullptr)
             {
                 hipCheck(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                enqueueStep(refine, nullptr);
+                try
+                {
+                    enqueueStep(refine, nullptr);
+                }
+                catch (...)
+                {
+                    // leave the stream out of capture mode before reporting
+                    hipGraph_t partial = nullptr;
+                    (void)hipStreamEndCapture(stream_, &partial);
+                    if (partial != nullptr)
+                    {
+                        (void)hipGraphDestroy(partial);
+                    }
+                    throw;
+                }
                 hipCheck(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
                 hipCheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
             }

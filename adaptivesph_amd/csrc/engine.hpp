@@ -34,6 +34,7 @@ public:
     // Returns the offset of a free range of n elements, or UINT64_MAX.
     uint64_t alloc(uint64_t n);
     void release(uint64_t off, uint64_t n);
+    void grow(uint64_t capacity);  // appends [capacity(), capacity) as free
     uint64_t capacity() const { return cap_; }
     uint64_t used() const { return used_; }
 
@@ -55,6 +56,8 @@ struct GridGeom
     uint32_t cells() const { return static_cast<uint32_t>(gx) * static_cast<uint32_t>(gy) * static_cast<uint32_t>(gz); }
 };
 GridGeom makeGrid(const asph_domain& d, float h0);
+// Throws Error(ASPH_ERR_INVALID) for parameters the pipeline cannot run.
+void validateParams(const asph_parameters& p);
 
 struct EngineConfig
 {
@@ -149,6 +152,10 @@ private:
     bool anyRefineNext() const;
     void advanceFrames();
     void invalidateGraphs();
+    template <class T>
+    void regrow(T*& p, size_t oldN, size_t newN);
+    void growCells(uint64_t newCap);
+    void growBoundary(uint64_t newCap);
     SceneState readState(uint32_t id);
 
     EngineConfig cfg_;

@@ -62,14 +62,17 @@ struct Ctx
     uint32_t cCap;
 
     // sorted shadow copies
-    F4* s0;  // (x, y, z, (2h)^2)
-    F4* s1;  // (vx, vy, vz, h)
-    float* sm;
-    F4* s2;  // (rho, p, p / rho^2, 0)
+    F4* s0;      // (x, y, z, (2h)^2)
+    F4* s1;      // (vx, vy, vz, m)
+    float2* shm;  // (h, m)
+    F4* s2;      // (rho, p, p / rho^2, h)
     uint32_t* sscene;
     uint32_t* ncount;
     uint32_t* list;  // [listCap * pCap], entry-major
     uint32_t listCap;
+    // per aligned group of kGroup sorted slots: bounding box and max (2h)^2
+    F4* gbox0;  // (min x, min y, min z, max (2h)^2)
+    F4* gbox1;  // (max x, max y, max z, 0)
 
     // boundary (sorted by cell per scene)
     F4* bs0;  // (x, y, z, psi)

@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
 // ------------------------------------------------- kick + drift + key --
 // halfKickVelocities (Algorithm.cu:270-284), updatePositions (:286-298),
 // assignParticlesToCells (NeighborKernels.cu:5-15) and the cell histogram.
-__global__ __launch_bounds__(256) void kKickDriftKey(Ctx c)
+__global__ __launch_bounds__(1024) void kKickDriftKey(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     for (uint32_t d = blockIdx.x * blockDim.x + threadIdx.x; d < n; d += gridDim.x * blockDim.x)
@@ -235,122 +235,381 @@ __global__ void kSceneRanges(Ctx c)
 }
 
 // --------------------------------------------------------------- gather --
-__global__ __launch_bounds__(256) void kGather(Ctx c)
+// Sorted shadow copies of the particle state and, per aligned group of
+// kGroup sorted slots, the bounding box of the group with the largest
+// (2h)^2 of its members (used to skip groups no member of which can pass the
+// neighbour test of a query).
+__global__ __launch_bounds__(1024) void kGather(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t nOrder = *c.nOrderPtr;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    const uint32_t nPad = (n + 63U) & ~63U;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nPad; t += gridDim.x * blockDim.x)
     {
-        const uint32_t k = segmentOf(c.sortedStarts, nOrder, t);
-        const uint32_t i = c.sortedVals[t];
-        const F4 p = c.pos[i];
-        const F4 v = c.vel[i];
-        const float hh = c.h[i];
-        const float r = 2.0F * hh;
-        c.s0[t] = F4 {p.x, p.y, p.z, r * r};
-        c.s1[t] = F4 {v.x, v.y, v.z, hh};
-        c.sm[t] = c.m[i];
-        c.sscene[t] = c.cellOrder[k];
+        float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
+        float hx = -FLT_MAX, hy = -FLT_MAX, hz = -FLT_MAX;
+        float mr = 0.0F;
+        if (t < n)
+        {
+            const uint32_t k = segmentOf(c.sortedStarts, nOrder, t);
+            const uint32_t i = c.sortedVals[t];
+            const F4 p = c.pos[i];
+            const F4 v = c.vel[i];
+            const float hh = c.h[i];
+            const float mm = c.m[i];
+            const float r = 2.0F * hh;
+            const float r2 = r * r;
+            c.s0[t] = F4 {p.x, p.y, p.z, r2};
+            c.s1[t] = F4 {v.x, v.y, v.z, mm};
+            c.shm[t] = float2 {hh, mm};
+            c.sscene[t] = c.cellOrder[k];
+            lx = hx = p.x;
+            ly = hy = p.y;
+            lz = hz = p.z;
+            mr = r2;
+        }
+        for (int off = 1; off < static_cast<int>(kGroup); off <<= 1)
+        {
+            lx = fminf(lx, __shfl_xor(lx, off, 64));
+            ly = fminf(ly, __shfl_xor(ly, off, 64));
+            lz = fminf(lz, __shfl_xor(lz, off, 64));
+            hx = fmaxf(hx, __shfl_xor(hx, off, 64));
+            hy = fmaxf(hy, __shfl_xor(hy, off, 64));
+            hz = fmaxf(hz, __shfl_xor(hz, off, 64));
+            mr = fmaxf(mr, __shfl_xor(mr, off, 64));
+        }
+        if ((t & (kGroup - 1U)) == 0U && t < n)
+        {
+            c.gbox0[t / kGroup] = F4 {lx, ly, lz, mr};
+            c.gbox1[t / kGroup] = F4 {hx, hy, hz, 0.0F};
+        }
+    }
+}
+
+// True unless every member of the group is provably farther from q than its
+// own support 2h (conservative by a relative margin far above FP32 rounding).
+__device__ __forceinline__ bool groupMayHit(const F4& b0, const F4& b1, const F4& q)
+{
+    const float ex = fmaxf(fmaxf(b0.x - q.x, q.x - b1.x), 0.0F);
+    const float ey = fmaxf(fmaxf(b0.y - q.y, q.y - b1.y), 0.0F);
+    const float ez = fmaxf(fmaxf(b0.z - q.z, q.z - b1.z), 0.0F);
+    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    return d2 <= b0.w * 1.0001F;
+}
+
+// Calls f(j, o) for every slot j in [lo, hi) (ascending) with o = s0[j] and
+// d2(q, o) <= o.w, visiting only groups that may hold such slots.
+template <class F>
+__device__ __forceinline__ void scanRange(const Ctx& c, uint32_t lo, uint32_t hi, const F4& q, F&& f)
+{
+    if (lo >= hi)
+    {
+        return;
+    }
+    for (uint32_t g = lo / kGroup; g <= (hi - 1U) / kGroup; ++g)
+    {
+        if (!groupMayHit(c.gbox0[g], c.gbox1[g], q))
+        {
+            continue;
+        }
+        const uint32_t base = g * kGroup;
+        F4 o[kGroup];
+#pragma unroll
+        for (uint32_t u = 0; u < kGroup; ++u)
+        {
+            o[u] = c.s0[min(base + u, hi - 1U)];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kGroup; ++u)
+        {
+            const uint32_t j = base + u;
+            if (j < lo || j >= hi)
+            {
+                continue;
+            }
+            const float dx = o[u].x - q.x;
+            const float dy = o[u].y - q.y;
+            const float dz = o[u].z - q.z;
+            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+            if (d2 <= o[u].w)
+            {
+                f(j, o[u], d2);
+            }
+        }
+    }
+}
+
+// Fluid walk of forEachFluidNeighbor (NeighborGrid.cuh:90-126) restricted to
+// accepted neighbours (d^2 <= (2 h_j)^2), in the reference's visiting order.
+template <class F>
+__device__ __forceinline__ void walkFluid(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
+                                          const F4& q, F&& f)
+{
+    const int gxgy = sc.gx * sc.gy;
+    for (int x = b.x0; x <= b.x1; ++x)
+    {
+        for (int y = b.y0; y <= b.y1; ++y)
+        {
+            for (int z = b.z0; z <= b.z1; ++z)
+            {
+                const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
+                scanRange(c, max(c.cellOff[cell], lost), c.cellOff[cell + 1U], q, f);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ bool touchesEdge(const SceneDev& sc, const CellBox& b)
+{
+    if ((sc.flags & kSceneBoundaryInside) != 0U)
+    {
+        return true;
+    }
+    return b.x0 == 0 || b.y0 == 0 || b.z0 == 0 || b.x1 == sc.gx - 1 || b.y1 == sc.gy - 1 || b.z1 == sc.gz - 1;
+}
+
+// ------------------------------------------------------- wave helpers --
+__device__ __forceinline__ int waveMinI(int v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = min(v, __shfl_xor(v, off, 64));
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ int waveMaxI(int v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = max(v, __shfl_xor(v, off, 64));
+    }
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ float waveMinF(float v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = fminf(v, __shfl_xor(v, off, 64));
+    }
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ __forceinline__ float waveMaxF(float v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = fmaxf(v, __shfl_xor(v, off, 64));
+    }
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+// Wave-uniform walk of the fluid grid for the 64 queries of a wave (all of one
+// scene). Every lane visits the union of the wave's candidate cells in the
+// reference's order (x, then y, then z innermost; slots ascending) and accepts
+// a candidate only if its cell lies in the lane's own candidate box and
+// d^2 <= (2 h_j)^2, so each lane sees exactly its reference neighbour
+// sequence. Candidates are read with wave-uniform addresses, and groups of
+// kGroup slots too far from every query of the wave are skipped.
+// Read-only views in the constant address space: loads with wave-uniform
+// addresses then go through the scalar cache (the arrays are not written by
+// the kernels that read them this way).
+typedef const __attribute__((address_space(4))) F4 ConstF4;
+typedef const __attribute__((address_space(4))) uint32_t ConstU32;
+
+__device__ __forceinline__ F4 ldc(ConstF4* p, uint32_t i) { return F4 {p[i].x, p[i].y, p[i].z, p[i].w}; }
+
+template <class F>
+__device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid, const CellBox& b,
+                                         const F4& q, F&& append)
+{
+    ConstF4* s0 = (ConstF4*)(c.s0);
+    ConstF4* gb0 = (ConstF4*)(c.gbox0);
+    ConstF4* gb1 = (ConstF4*)(c.gbox1);
+    ConstU32* off = (ConstU32*)(c.cellOff);
+    const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
+    const int uy0 = waveMinI(valid ? b.y0 : 0x7FFFFFFF);
+    const int uz0 = waveMinI(valid ? b.z0 : 0x7FFFFFFF);
+    const int ux1 = waveMaxI(valid ? b.x1 : -1);
+    const int uy1 = waveMaxI(valid ? b.y1 : -1);
+    const int uz1 = waveMaxI(valid ? b.z1 : -1);
+    const float wx0 = waveMinF(valid ? q.x : FLT_MAX);
+    const float wy0 = waveMinF(valid ? q.y : FLT_MAX);
+    const float wz0 = waveMinF(valid ? q.z : FLT_MAX);
+    const float wx1 = waveMaxF(valid ? q.x : -FLT_MAX);
+    const float wy1 = waveMaxF(valid ? q.y : -FLT_MAX);
+    const float wz1 = waveMaxF(valid ? q.z : -FLT_MAX);
+    const int gx = sc.gx;
+    const int gxgy = sc.gx * sc.gy;
+    const uint32_t base = sc.cellBase;
+    for (int x = ux0; x <= ux1; ++x)
+    {
+        for (int y = uy0; y <= uy1; ++y)
+        {
+            for (int z = uz0; z <= uz1; ++z)
+            {
+                const bool inbox =
+                    valid && x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1;
+                if (__ballot(inbox ? 1 : 0) == 0ULL)
+                {
+                    continue;
+                }
+                const uint32_t cell = base + static_cast<uint32_t>(x + y * gx + z * gxgy);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
+                const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+                if (lo >= hi)
+                {
+                    continue;
+                }
+                const uint32_t g1 = (hi - 1U) / kGroup;
+                for (uint32_t g = lo / kGroup; g <= g1; ++g)
+                {
+                    const F4 b0 = ldc(gb0, g);
+                    const F4 b1 = ldc(gb1, g);
+                    const float ex = fmaxf(fmaxf(b0.x - wx1, wx0 - b1.x), 0.0F);
+                    const float ey = fmaxf(fmaxf(b0.y - wy1, wy0 - b1.y), 0.0F);
+                    const float ez = fmaxf(fmaxf(b0.z - wz1, wz0 - b1.z), 0.0F);
+                    const float d2g = (ex * ex + ey * ey) + ez * ez;
+                    if (!(d2g <= b0.w * 1.0001F))
+                    {
+                        continue;
+                    }
+                    const uint32_t gb = g * kGroup;
+                    F4 o[kGroup];
+#pragma unroll
+                    for (uint32_t u = 0; u < kGroup; ++u)
+                    {
+                        o[u] = ldc(s0, gb + u < hi ? gb + u : hi - 1U);
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < kGroup; ++u)
+                    {
+                        const uint32_t j = gb + u;
+                        if (j < lo || j >= hi)
+                        {
+                            continue;
+                        }
+                        const float dx = o[u].x - q.x;
+                        const float dy = o[u].y - q.y;
+                        const float dz = o[u].z - q.z;
+                        const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                        if (inbox && d2 <= o[u].w)
+                        {
+                            append(j);
+                        }
+                    }
+                }
+            }
+        }
     }
 }
 
 // -------------------------------------------------------------- density --
 // computeDensities (Algorithm.cu:17-54) and computeBoundaryDensityContribution
-// (BoundaryAlgorithm.cu:15-51). Phase 1 walks the candidate cells in the
-// reference's order (x, then y, then z innermost; slots ascending) and stores
-// the accepted slots in an entry-major neighbour list; phase 2 accumulates the
-// density over that list with every lane active.
+// (BoundaryAlgorithm.cu:15-51). Phase 1 finds each particle's accepted
+// neighbours in the reference's visiting order and stores their slots in an
+// entry-major neighbour list; phase 2 accumulates the density over that list
+// in the same order with every lane active.
 __global__ __launch_bounds__(256) void kDensity(Ctx c)
 {
     const uint32_t n = *c.nTotal;
+    const uint32_t nPad = (n + 63U) & ~63U;
     const uint32_t K = c.listCap;
-    const uint32_t LS = c.pCap;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    const size_t LS = c.pCap;
+    for (uint32_t tw = blockIdx.x * blockDim.x + threadIdx.x; tw < nPad; tw += gridDim.x * blockDim.x)
     {
+        const bool valid = tw < n;
+        const uint32_t t = valid ? tw : n - 1U;
         const uint32_t s = c.sscene[t];
+        const uint32_t sw = __builtin_amdgcn_readfirstlane(s);
+        const bool oneScene = __ballot((valid && s != sw) ? 1 : 0) == 0ULL;
         const SceneDev& sc = c.scenes[s];
         const uint32_t lost = c.states[s].lostEnd;
         const F4 q = c.s0[t];
-        const float hi = c.s1[t].w;
+        const float hi = c.shm[t].x;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        const int gxgy = sc.gx * sc.gy;
         uint32_t cnt = 0;
-        for (int x = b.x0; x <= b.x1; ++x)
+        uint32_t* lst = c.list + t;
+        if (oneScene)
         {
-            for (int y = b.y0; y <= b.y1; ++y)
-            {
-                for (int z = b.z0; z <= b.z1; ++z)
+            waveWalk(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, [&](uint32_t j) {
+                if (cnt < K)
                 {
-                    const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                    const uint32_t lo = max(c.cellOff[cell], lost);
-                    const uint32_t hiS = c.cellOff[cell + 1U];
-                    for (uint32_t j = lo; j < hiS; ++j)
-                    {
-                        const F4 o = c.s0[j];
-                        const float dx = o.x - q.x;
-                        const float dy = o.y - q.y;
-                        const float dz = o.z - q.z;
-                        const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                        if (d2 <= o.w)
-                        {
-                            if (cnt < K)
-                            {
-                                c.list[static_cast<size_t>(cnt) * LS + t] = j;
-                            }
-                            ++cnt;
-                        }
-                    }
+                    lst[cnt * LS] = j;
                 }
-            }
+                ++cnt;
+            });
+        }
+        else if (valid)
+        {
+            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
+                if (cnt < K)
+                {
+                    lst[cnt * LS] = j;
+                }
+                ++cnt;
+            });
+        }
+        if (!valid)
+        {
+            continue;
         }
         float rho = 0.0F;
         if (cnt <= K)
         {
-            for (uint32_t e = 0; e < cnt; ++e)
+            uint32_t e = 0;
+            for (; e + 4U <= cnt; e += 4U)
             {
-                const uint32_t j = c.list[static_cast<size_t>(e) * LS + t];
-                const F4 o = c.s0[j];
+                uint32_t j[4];
+                F4 o[4];
+                float2 hm[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    j[u] = lst[(e + u) * LS];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    o[u] = c.s0[j[u]];
+                    hm[u] = c.shm[j[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                {
+                    const float dx = o[u].x - q.x;
+                    const float dy = o[u].y - q.y;
+                    const float dz = o[u].z - q.z;
+                    const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
+                    rho = rho + hm[u].y * wendland(d, hm[u].x);
+                }
+            }
+            for (; e < cnt; ++e)
+            {
+                const uint32_t jj = lst[e * LS];
+                const F4 o = c.s0[jj];
+                const float2 hm = c.shm[jj];
                 const float dx = o.x - q.x;
                 const float dy = o.y - q.y;
                 const float dz = o.z - q.z;
-                const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                const float d = sqrtf(d2);
-                rho = rho + c.sm[j] * wendland(d, c.s1[j].w);
+                rho = rho + hm.y * wendland(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), hm.x);
             }
         }
         else
         {
-            for (int x = b.x0; x <= b.x1; ++x)
-            {
-                for (int y = b.y0; y <= b.y1; ++y)
-                {
-                    for (int z = b.z0; z <= b.z1; ++z)
-                    {
-                        const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                        const uint32_t lo = max(c.cellOff[cell], lost);
-                        const uint32_t hiS = c.cellOff[cell + 1U];
-                        for (uint32_t j = lo; j < hiS; ++j)
-                        {
-                            const F4 o = c.s0[j];
-                            const float dx = o.x - q.x;
-                            const float dy = o.y - q.y;
-                            const float dz = o.z - q.z;
-                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            if (d2 <= o.w)
-                            {
-                                rho = rho + c.sm[j] * wendland(sqrtf(d2), c.s1[j].w);
-                            }
-                        }
-                    }
-                }
-            }
+            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float d2) {
+                const float2 hm = c.shm[j];
+                rho = rho + hm.y * wendland(sqrtf(d2), hm.x);
+            });
         }
         // boundary contribution (always added, as the reference launches it unconditionally)
         float rhoB = 0.0F;
-        if (sc.bCount > 0U)
+        if (sc.bCount > 0U && touchesEdge(sc, b))
         {
             const float r2 = R * R;
+            const int gxgy = sc.gx * sc.gy;
             for (int x = b.x0; x <= b.x1; ++x)
             {
                 for (int y = b.y0; y <= b.y1; ++y)
@@ -378,7 +637,7 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
         }
         rho = rho + rhoB;
         const float p = tait(rho, sc);
-        c.s2[t] = F4 {rho, p, p / (rho * rho), 0.0F};
+        c.s2[t] = F4 {rho, p, p / (rho * rho), hi};
         c.ncount[t] = cnt;
     }
 }
@@ -394,15 +653,15 @@ struct PairAcc
 // One fluid pair of computePressureAccelerations (Algorithm.cu:72-105),
 // computeViscosityAccelerations (:129-171) and
 // computeSurfaceTensionAccelerations (:192-218), accumulated separately.
-__device__ __forceinline__ void forcePair(const Ctx& c, uint32_t j, const F4& q, const F4& vi, float rhoi, float prri,
-                                          float nuNum, float epsH2, bool doVisc, bool doST, PairAcc& A)
+// o = s0[j], nj = s1[j] (velocity, mass), rj = s2[j] (rho, p, p/rho^2, h).
+__device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& rj, const F4& q, const F4& vi,
+                                          float rhoi, float prri, float nuNum, float epsH2, bool doVisc, bool doST,
+                                          PairAcc& A)
 {
-    const F4 o = c.s0[j];
     const float ox = q.x - o.x;
     const float oy = q.y - o.y;
     const float oz = q.z - o.z;
     const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
-    const F4 rj = c.s2[j];
     const float d = sqrtf(d2);
     float dx = 0.0F;
     float dy = 1.0F;
@@ -413,10 +672,9 @@ __device__ __forceinline__ void forcePair(const Ctx& c, uint32_t j, const F4& q,
         dy = oy / d;
         dz = oz / d;
     }
-    const F4 nj = c.s1[j];
-    const float mj = c.sm[j];
+    const float mj = nj.w;
     const float pterm = prri + rj.z;
-    const float dw = wendlandGrad(d, nj.w);
+    const float dw = wendlandGrad(d, rj.w);
     A.apx = A.apx - ((mj * dx) * dw) * pterm;
     A.apy = A.apy - ((mj * dy) * dw) * pterm;
     A.apz = A.apz - ((mj * dz) * dw) * pterm;
@@ -438,7 +696,7 @@ __device__ __forceinline__ void forcePair(const Ctx& c, uint32_t j, const F4& q,
     }
     if (doST)
     {
-        const float sw = mj * wendland(d, nj.w);
+        const float sw = mj * wendland(d, rj.w);
         A.asx = A.asx + sw * dx;
         A.asy = A.asy + sw * dy;
         A.asz = A.asz + sw * dz;
@@ -453,7 +711,7 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t K = c.listCap;
-    const uint32_t LS = c.pCap;
+    const size_t LS = c.pCap;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
         const uint32_t s = c.sscene[t];
@@ -461,8 +719,8 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         const SceneState& st = c.states[s];
         const F4 q = c.s0[t];
         const F4 vi = c.s1[t];
-        const float hi = vi.w;
         const F4 ri = c.s2[t];
+        const float hi = ri.w;
         const float rhoi = ri.x;
         const float prri = ri.z;
         const uint32_t cnt = c.ncount[t];
@@ -472,50 +730,47 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         const float epsH2 = (0.01F * hi) * hi;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        const int gxgy = sc.gx * sc.gy;
         PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
         if (cnt <= K)
         {
-            for (uint32_t e = 0; e < cnt; ++e)
+            const uint32_t* lst = c.list + t;
+            uint32_t e = 0;
+            for (; e + 2U <= cnt; e += 2U)
             {
-                const uint32_t j = c.list[static_cast<size_t>(e) * LS + t];
-                if (j != t)
+                const uint32_t j0 = lst[e * LS];
+                const uint32_t j1 = lst[(e + 1U) * LS];
+                const F4 o0 = c.s0[j0];
+                const F4 n0 = c.s1[j0];
+                const F4 r0 = c.s2[j0];
+                const F4 o1 = c.s0[j1];
+                const F4 n1 = c.s1[j1];
+                const F4 r1 = c.s2[j1];
+                if (j0 != t)
                 {
-                    forcePair(c, j, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+                    forcePair(o0, n0, r0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+                }
+                if (j1 != t)
+                {
+                    forcePair(o1, n1, r1, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+                }
+            }
+            if (e < cnt)
+            {
+                const uint32_t j0 = lst[e * LS];
+                if (j0 != t)
+                {
+                    forcePair(c.s0[j0], c.s1[j0], c.s2[j0], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
                 }
             }
         }
         else
         {
-            const uint32_t lost = st.lostEnd;
-            for (int x = b.x0; x <= b.x1; ++x)
-            {
-                for (int y = b.y0; y <= b.y1; ++y)
+            walkFluid(c, sc, st.lostEnd, b, q, [&](uint32_t j, const F4& o, float) {
+                if (j != t)
                 {
-                    for (int z = b.z0; z <= b.z1; ++z)
-                    {
-                        const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                        const uint32_t lo = max(c.cellOff[cell], lost);
-                        const uint32_t hiS = c.cellOff[cell + 1U];
-                        for (uint32_t j = lo; j < hiS; ++j)
-                        {
-                            if (j == t)
-                            {
-                                continue;
-                            }
-                            const F4 o = c.s0[j];
-                            const float dx = q.x - o.x;
-                            const float dy = q.y - o.y;
-                            const float dz = q.z - o.z;
-                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            if (d2 <= o.w)
-                            {
-                                forcePair(c, j, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                            }
-                        }
-                    }
+                    forcePair(o, c.s1[j], c.s2[j], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
                 }
-            }
+            });
         }
         float ax = st.gEff[0];
         float ay = st.gEff[1];
@@ -531,16 +786,17 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         }
         if (doST)
         {
-            const float k = sc.sigma / c.sm[t];
+            const float k = sc.sigma / vi.w;
             ax = ax + k * A.asx;
             ay = ay + k * A.asy;
             az = az + k * A.asz;
         }
         float bpx = 0.F, bpy = 0.F, bpz = 0.F;
         float bfx = 0.F, bfy = 0.F, bfz = 0.F;
-        if (sc.bCount > 0U)
+        if (sc.bCount > 0U && touchesEdge(sc, b))
         {
             const float r2 = R * R;
+            const int gxgy = sc.gx * sc.gy;
             for (int x = b.x0; x <= b.x1; ++x)
             {
                 for (int y = b.y0; y <= b.y1; ++y)

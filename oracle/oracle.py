@@ -67,6 +67,10 @@ def load() -> C.CDLL:
         "oracle_sincos": (None, [f64, P(f64), P(f64)]),
         "oracle_mix64": (C.c_uint64, [C.c_uint64]),
         "oracle_boundary_generate": (u32, [P(abi.Domain), f32, f32, f32, vp]),
+        "oracle_split_one": (C.c_int, [vp, u32, u32]),
+        "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32]),
+        "oracle_top_list": (u32, [vp, u32, C.c_int, f32, vp]),
+        "oracle_cell_ranges": (None, [vp, u32, u32, u32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -170,6 +174,10 @@ class OracleSim:
         if rc != 0:
             raise RuntimeError("oracle_set_fluid failed")
 
+    def split_one(self, parent: int, base: int) -> None:
+        if self._lib.oracle_split_one(self._h, int(parent), int(base)) != 0:
+            raise RuntimeError("oracle_split_one: bad index")
+
     def grid_dims(self) -> tuple[int, int, int]:
         d = (C.c_int32 * 3)()
         self._lib.oracle_grid_dims(self._h, d)
@@ -222,6 +230,28 @@ class OracleSim:
         out["pair_keep"] = keep[: t.merge_pairs]
         out["split_parents"] = split[: t.split_performed]
         return out
+
+
+def top_list(values: np.ndarray, descending: bool, ratio: float) -> np.ndarray:
+    v = np.ascontiguousarray(values, np.float32)
+    out = np.zeros(max(len(v), 1), np.uint32)
+    k = load().oracle_top_list(v.ctypes.data, len(v), int(descending), ratio, out.ctypes.data)
+    return out[:k]
+
+
+def cell_ranges(sorted_keys: np.ndarray, cells: int, fix: bool = False) -> tuple[np.ndarray, np.ndarray]:
+    k = np.ascontiguousarray(sorted_keys, np.int32)
+    s = np.zeros(cells, np.int32)
+    e = np.zeros(cells, np.int32)
+    load().oracle_cell_ranges(k.ctypes.data, len(k), cells, int(fix), s.ctypes.data, e.ctypes.data)
+    return s, e
+
+
+def merged_h(pa, pb, ma, mb, ha, hb, pm, M) -> float:
+    a = np.asarray(pa, np.float32)
+    b = np.asarray(pb, np.float32)
+    m = np.asarray(pm, np.float32)
+    return float(load().oracle_merged_h(a.ctypes.data, b.ctypes.data, ma, mb, ha, hb, m.ctypes.data, M))
 
 
 def wendland(d, h, precision=64):

@@ -1769,6 +1769,95 @@ void oracle_refinement_trace(oracle_sim* s, asph_refinement_trace* t, uint32_t* 
     });
 }
 
+// Split particle `parent` into slots [base, base + 12) of the current state
+// (splitParticles body, AdaptiveAlgorithm.cu:88-116); count grows to base + 12.
+int oracle_split_one(oracle_sim* s, uint32_t parent, uint32_t base)
+{
+    int rc = 0;
+    ORACLE_DISPATCH(s, {
+        if (parent >= S.n || base + 12U > S.cap)
+        {
+            rc = 1;
+        }
+        else
+        {
+            S.splitOne(parent, base);
+            S.n = std::max(S.n, base + 12U);
+        }
+    });
+    return rc;
+}
+
+// Merged smoothing length of two particles (calculateMergedSmoothingLength).
+float oracle_merged_h(const float* pa, const float* pb, float ma, float mb, float ha, float hb, const float* pm,
+                      float M)
+{
+    oracle::Sim<float> tmp;
+    const oracle::V4<float> a {pa[0], pa[1], pa[2], pa[3]};
+    const oracle::V4<float> b {pb[0], pb[1], pb[2], pb[3]};
+    const oracle::V4<float> m {pm[0], pm[1], pm[2], pm[3]};
+    return tmp.mergedSmoothingLength(a, b, ma, mb, ha, hb, m, M);
+}
+
+// findTopParticlesToSplit / ToMerge selection on a value array.
+uint32_t oracle_top_list(const float* values, uint32_t n, int descending, float ratio, uint32_t* out)
+{
+    oracle::Sim<float> tmp;
+    tmp.ref.batchRatio = ratio;
+    const std::vector<float> v(values, values + n);
+    uint32_t k = 0;
+    const auto ids = tmp.topList(v, descending != 0, &k);
+    std::copy(ids.begin(), ids.end(), out);
+    return k;
+}
+
+// calculateCellStartAndEndIndices on given sorted keys (cells entries each).
+void oracle_cell_ranges(const int32_t* sortedKeys, uint32_t n, uint32_t cells, uint32_t fixQuirk, int32_t* start,
+                        int32_t* end)
+{
+    oracle::Sim<float> tmp;
+    tmp.p.flags = fixQuirk != 0U ? ASPH_FLAG_FIX_CELL_START : 0U;
+    tmp.gx = static_cast<int>(cells);
+    tmp.gy = 1;
+    tmp.gz = 1;
+    // reuse buildGrid's range rule on keys already sorted
+    oracle::Grid g;
+    g.count = n;
+    g.cellStart.assign(cells, -1);
+    g.cellEnd.assign(cells, -1);
+    g.keys.assign(sortedKeys, sortedKeys + n);
+    const bool fix = fixQuirk != 0U;
+    for (uint32_t sl = 0; sl < n; ++sl)
+    {
+        const int32_t c = g.keys[sl];
+        if (sl == n - 1)
+        {
+            g.cellEnd[c] = static_cast<int32_t>(sl);
+            if (fix && sl == 0)
+            {
+                g.cellStart[c] = 0;
+            }
+            continue;
+        }
+        if (sl == 0)
+        {
+            g.cellStart[c] = 0;
+            if (!fix)
+            {
+                continue;
+            }
+        }
+        const int32_t cn = g.keys[sl + 1];
+        if (c != cn)
+        {
+            g.cellStart[cn] = static_cast<int32_t>(sl + 1);
+            g.cellEnd[c] = static_cast<int32_t>(sl);
+        }
+    }
+    std::copy(g.cellStart.begin(), g.cellStart.end(), start);
+    std::copy(g.cellEnd.begin(), g.cellEnd.end(), end);
+}
+
 // Scalar closed forms for the math pins.
 float oracle_wendland_f32(float d, float h) { return oracle::wendland<float>(d, h); }
 double oracle_wendland_f64(double d, double h) { return oracle::wendland<double>(d, h); }

@@ -1,0 +1,143 @@
+"""Host-side checks of the C ABI (no GPU needed): every symbol declared in
+include/*.h is exported, struct layouts agree with a C compiler, defaults
+mirror the reference, argument validation fails before any device call, and
+the product's boundary generator equals the oracle's restatement."""
+import ctypes as C
+import re
+import subprocess
+import textwrap
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from adaptivesph_amd import abi, scene
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _declared():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b(asph_\w+)\s*\(", h.read_text(), re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_all_declared_symbols_exported():
+    lib = abi.load_library()
+    names = _declared()
+    assert len(names) >= 35
+    out = subprocess.run(["nm", "-D", "--defined-only", str(abi.LIB_PATH)], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert names <= exported, sorted(names - exported)
+    for n in names:
+        getattr(lib, n)
+
+
+def test_struct_layout_matches_c(tmp_path):
+    structs = {
+        "asph_parameters": abi.Parameters,
+        "asph_refinement_parameters": abi.RefinementParameters,
+        "asph_density_info": abi.DensityInfo,
+        "asph_boundary_particle": abi.BoundaryParticle,
+        "asph_fluid_view": abi.FluidView,
+        "asph_batch_config": abi.BatchConfig,
+        "asph_refinement_trace": abi.RefinementTrace,
+        "asph_domain": abi.Domain,
+    }
+    src = ["#include <stdio.h>", "#include <stddef.h>", '#include "asph_debug.h"', '#include "asph_batch.h"',
+           "int main(void){"]
+    for cname, py in structs.items():
+        src.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            src.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    src.append("return 0;}")
+    c = tmp_path / "layout.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", str(ROOT / "include"), str(c), "-o", str(exe)], check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for cname, py in structs.items():
+        assert int(got[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(py, f).offset, f"{cname}.{f}"
+    # the mirrored prefix of Simulation::Parameters is 80 bytes (Simulation.cuh:64-109)
+    assert abi.Parameters.tait_gamma.offset == 80
+
+
+def test_defaults_mirror_reference():
+    lib = abi.load_library()
+    p = abi.Parameters()
+    assert lib.asph_default_parameters(C.byref(p)) == 0
+    assert (p.domain.min.x, p.domain.max.y, p.domain.friction, p.threads_per_block) == (-1.0, 1.0,
+                                                                                        pytest.approx(0.01), 256)
+    assert p.tait_gamma == 5.0
+    r = abi.RefinementParameters()
+    assert lib.asph_default_refinement_parameters(C.byref(r)) == 0
+    want = abi.default_refinement()
+    for f, _ in abi.RefinementParameters._fields_:
+        a, b = getattr(r, f), getattr(want, f)
+        if isinstance(a, C.Structure):
+            assert bytes(a) == bytes(b), f
+        else:
+            assert a == pytest.approx(b), f
+    assert (r.min_mass_ratio, r.max_mass_ratio, r.max_particle_count, r.initial_cooldown, r.cooldown) == (
+        pytest.approx(0.3), 8.0, 1000000, 10, 10)
+    assert (r.splitting.epsilon, r.splitting.alpha, r.splitting.center_mass_ratio) == (
+        pytest.approx(0.65), pytest.approx(0.7), pytest.approx(0.077))
+
+
+def test_validation_before_device_calls():
+    lib = abi.load_library()
+    s = scene.dam_break((4, 4, 4), 0.01, (0.2, 0.2, 0.2), seed=1, steps=1, boundary="")
+    bad = abi.Parameters.from_buffer_copy(s.params)
+    bad.base_smoothing_radius = 0.0
+    h = C.c_void_p()
+    rc = lib.asph_create(C.byref(bad), s.positions.ctypes.data, s.count, None, 0, None, s.count, 1, C.byref(h))
+    assert rc == 1 and b"smoothing" in lib.asph_last_error()
+    rc = lib.asph_create(C.byref(s.params), s.positions.ctypes.data, s.count, None, 0, None, s.count - 1, 1,
+                         C.byref(h))
+    assert rc == 2 and b"capacity" in lib.asph_last_error()
+    tiny = abi.Parameters.from_buffer_copy(s.params)
+    tiny.base_smoothing_radius = 1e-6
+    rc = lib.asph_create(C.byref(tiny), s.positions.ctypes.data, s.count, None, 0, None, s.count, 1, C.byref(h))
+    assert rc == 1 and b"cells" in lib.asph_last_error()
+    assert lib.asph_update(None, 0.1) == 1
+
+
+def test_grid_cells_of_the_workloads():
+    assert scene.cfg1().meta["n"] == (25, 50, 25)
+    from adaptivesph_amd.batch import grid_cells
+
+    assert grid_cells(scene.dam_break((2, 2, 2), 0.01, (1.0, 0.6, 0.5), boundary="").params) == 20 * 12 * 10
+    assert grid_cells(scene.dam_break((2, 2, 2), 0.0039, (2.0, 1.2, 1.0), boundary="").params) == 99 * 60 * 50
+
+
+@pytest.mark.parametrize("which,count", [("cfg1", 37482), ("cfg2", 851860)])
+def test_boundary_generator_matches_oracle(which, count):
+    s = getattr(scene, which)(boundary="")
+    lib_b = scene.generate_boundary(s.params, "library")
+    ora_b = scene.generate_boundary(s.params, "oracle")
+    assert len(lib_b) == count
+    assert np.array_equal(lib_b.view(np.uint8), ora_b.view(np.uint8))
+
+
+def test_splitmix_uniform_known_value():
+    u = scene.splitmix64_uniform(0, 1)[0]
+    assert u == (0xE220A8397B1DCDAF >> 40) * 2.0 ** -24
+    a = scene.lattice((3, 4, 5), 0.01, (0.0, 0.0, 0.0), seed=9)
+    b = scene.lattice((3, 4, 5), 0.01, (0.0, 0.0, 0.0), seed=9)
+    assert np.array_equal(a, b) and a.shape == (60, 4) and (a[:, 3] == 0).all()
+    # x outer, z innermost (App.cpp:326-339)
+    assert a[1, 2] > a[0, 2] and abs(a[1, 0] - a[0, 0]) < 0.001
+
+
+def test_scene_bytes_monotone():
+    lib = abi.load_library()
+    a = lib.asph_scene_bytes(1000, 0, 100, 0, 128)
+    b = lib.asph_scene_bytes(2000, 0, 100, 0, 128)
+    c = lib.asph_scene_bytes(2000, 0, 100, 1, 128)
+    assert 0 < a < b < c

@@ -1,0 +1,134 @@
+"""GPU parity of the adaptive step (split / merge) against the FP32 oracle.
+
+On identical FP32 inputs the merge list, merge candidates, merged pairs, the
+split parents and the whole particle state after compaction and splitting
+must equal the oracle's bits, for every criterion. Daughter slots are
+assigned in split-list order (the reference assigns them by atomicAdd, in an
+unspecified order); the oracle uses the same rule.
+"""
+import numpy as np
+import pytest
+
+from adaptivesph_amd import scene
+from adaptivesph_amd.simulation import Simulation
+from helpers import assert_bitexact, floor_block_scene, random_state, refinement
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _adaptive_pair(r, seed=21, n=(10, 12, 8), vmax=0.1):
+    s = floor_block_scene(n, seed=seed)
+    s.refinement = r
+    s.capacity = r.max_particle_count
+    st = random_state(s, seed + 1, vmax)
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    sim.set_fluid(dict(st))
+    ref.set_fluid(dict(st))
+    return s, sim, ref
+
+
+def _compare_traces(a, b, where):
+    for k in ("refined", "count_before", "merge_eligible", "merge_pairs", "count_after_merge", "split_selected",
+              "split_performed", "count_after"):
+        assert a[k] == b[k], f"{where}: {k} {a[k]} vs {b[k]}"
+    for k in ("merge_list", "candidates", "pair_keep", "split_parents"):
+        assert np.array_equal(a[k], b[k]), f"{where}: {k}"
+
+
+CASES = {
+    "velocity": dict(velocity=(0.03, 0.2)),
+    "interface": dict(interface=(0.1, 0.15)),
+    "vorticity": dict(vorticity=(5.0, 20.0)),
+    "zone": dict(zone=((0.15, 0.06, 0.1), 0.04)),
+}
+
+
+@pytest.mark.parametrize("criterion", list(CASES))
+def test_refinement_bitexact(criterion):
+    r = refinement(criterion, 20000, **CASES[criterion])
+    s, sim, ref = _adaptive_pair(r)
+    merged = split = 0
+    for step in range(1, 7):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        ta, tb = sim.refinement_trace(), ref.refinement_trace()
+        _compare_traces(ta, tb, f"{criterion} step {step}")
+        merged += ta["merge_pairs"]
+        split += ta["split_performed"]
+        assert sim.get_fluid_particles_count() == ref.count()
+        assert_bitexact(sim.fluid(), ref.fluid(), f"{criterion} step {step}")
+    assert merged > 0, "the case must exercise merging"
+    if criterion in ("velocity", "interface", "zone"):
+        assert split > 0, "the case must exercise splitting"
+
+
+def test_split_capacity_exhaustion():
+    """Only whole icosahedral splits that fit below maxParticleCount happen."""
+    base = 10 * 12 * 8
+    r = refinement("velocity", base + 30, velocity=(0.03, 0.2))
+    s, sim, ref = _adaptive_pair(r)
+    for step in range(1, 4):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        _compare_traces(sim.refinement_trace(), ref.refinement_trace(), f"cap step {step}")
+        assert_bitexact(sim.fluid(), ref.fluid(), f"cap step {step}")
+        assert sim.get_fluid_particles_count() <= base + 30
+
+
+def test_cadence_and_enable():
+    """Refinement runs at frame == initialCooldown and every cooldown frames
+    after (AdaptiveSphSimulation.cu:75-79); enableAdaptiveRefinement resets
+    the frame counter to initialCooldown."""
+    r = refinement("velocity", 20000, velocity=(0.03, 0.2), initial_cooldown=3, cooldown=2)
+    s, sim, ref = _adaptive_pair(r)
+    ran = []
+    for step in range(8):
+        if step == 5:
+            sim.enable_adaptive_refinement()
+            ref.enable_refinement()
+        sim.update(s.dt)
+        ref.update(s.dt)
+        t = sim.refinement_trace()
+        _compare_traces(t, ref.refinement_trace(), f"cadence {step}")
+        ran.append(t["refined"])
+    assert ran == [0, 0, 0, 1, 1, 1, 1, 0]
+    assert_bitexact(sim.fluid(), ref.fluid(), "cadence")
+
+
+def test_merge_selection_takes_sentinels_first():
+    """findTopParticlesToMerge sorts ascending, so heavy particles (criterion -1)
+    fill the list before eligible ones (reference quirk, kept)."""
+    r = refinement("velocity", 20000, velocity=(0.03, 0.2), max_mass_ratio=1.5)
+    s, sim, ref = _adaptive_pair(r, seed=31)
+    st = random_state(s, 32, 0.01)
+    m = np.full(s.count, s.params.base_particle_mass, np.float32)
+    m[::7] *= 2.0  # heavier than max_mass_ratio * m0 -> criterion -1
+    st["masses"] = m
+    sim.set_fluid(dict(st))
+    ref.set_fluid(dict(st))
+    sim.update(s.dt)
+    ref.update(s.dt)
+    ta = sim.refinement_trace()
+    _compare_traces(ta, ref.refinement_trace(), "sentinels")
+    heavy = set(range(0, s.count, 7))
+    assert set(ta["merge_list"][: len(heavy)].tolist()) == heavy
+    assert_bitexact(sim.fluid(), ref.fluid(), "sentinels")
+
+
+def test_adaptive_long_run_zone_slosh():
+    """A small adaptive sloshing tank with the moving zone, 60 steps."""
+    s = scene.sloshing_tank((16, 8, 10), 0.01, (0.25, 0.2, 0.15), seed=8, steps=60, max_count=40000)
+    s.refinement.zone.radius = 0.05
+    s.refinement.zone.path_radius = 0.04
+    s.refinement.zone.path_period = 0.01
+    s.refinement.zone.center.y = 0.05
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    for step in range(1, s.steps + 1):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        _compare_traces(sim.refinement_trace(), ref.refinement_trace(), f"slosh step {step}")
+        if step % 20 == 0:
+            assert_bitexact(sim.fluid(), ref.fluid(), f"slosh step {step}")
