@@ -247,7 +247,7 @@ Engine::Engine(const EngineConfig& cfg)
         c.sscene = alloc<uint32_t>(P);
         c.ncount = alloc<uint32_t>(P);
         c.listCap = cfg_.listCap;
-        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap) * P);
+        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap + 1U) * P);  // + spare row
         c.bs0 = alloc<F4>(B);
         c.bvisc = alloc<float>(B);
         c.bcellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);

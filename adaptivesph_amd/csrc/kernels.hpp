@@ -68,7 +68,7 @@ struct Ctx
     F4* s2;      // (rho, p, p / rho^2, h)
     uint32_t* sscene;
     uint32_t* ncount;
-    uint32_t* list;  // [listCap * pCap], entry-major
+    uint32_t* list;  // [(listCap + 1) * pCap], entry-major; row listCap is spare
     uint32_t listCap;
     // per aligned group of kGroup sorted slots: bounding box and max (2h)^2
     F4* gbox0;  // (min x, min y, min z, max (2h)^2)

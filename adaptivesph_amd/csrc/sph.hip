@@ -476,27 +476,43 @@ __device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint3
                         continue;
                     }
                     const uint32_t gb = g * kGroup;
-                    F4 o[kGroup];
-#pragma unroll
-                    for (uint32_t u = 0; u < kGroup; ++u)
+                    if (gb >= lo && gb + kGroup <= hi)
                     {
-                        o[u] = ldc(s0, gb + u < hi ? gb + u : hi - 1U);
-                    }
+                        // whole group inside the cell range
+                        F4 o[kGroup];
 #pragma unroll
-                    for (uint32_t u = 0; u < kGroup; ++u)
-                    {
-                        const uint32_t j = gb + u;
-                        if (j < lo || j >= hi)
+                        for (uint32_t u = 0; u < kGroup; ++u)
                         {
-                            continue;
+                            o[u] = ldc(s0, gb + u);
                         }
-                        const float dx = o[u].x - q.x;
-                        const float dy = o[u].y - q.y;
-                        const float dz = o[u].z - q.z;
-                        const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                        if (inbox && d2 <= o[u].w)
+#pragma unroll
+                        for (uint32_t u = 0; u < kGroup; ++u)
                         {
-                            append(j);
+                            const float dx = o[u].x - q.x;
+                            const float dy = o[u].y - q.y;
+                            const float dz = o[u].z - q.z;
+                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                            if (inbox && d2 <= o[u].w)
+                            {
+                                append(gb + u);
+                            }
+                        }
+                    }
+                    else
+                    {
+                        const uint32_t jlo = max(lo, gb);
+                        const uint32_t jhi = min(hi, gb + kGroup);
+                        for (uint32_t j = jlo; j < jhi; ++j)
+                        {
+                            const F4 o = ldc(s0, j);
+                            const float dx = o.x - q.x;
+                            const float dy = o.y - q.y;
+                            const float dz = o.z - q.z;
+                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                            if (inbox && d2 <= o.w)
+                            {
+                                append(j);
+                            }
                         }
                     }
                 }
@@ -511,7 +527,7 @@ __device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint3
 // neighbours in the reference's visiting order and stores their slots in an
 // entry-major neighbour list; phase 2 accumulates the density over that list
 // in the same order with every lane active.
-__global__ __launch_bounds__(256) void kDensity(Ctx c)
+__global__ __launch_bounds__(256) void kNeighbors(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t nPad = (n + 63U) & ~63U;
@@ -535,10 +551,7 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
         if (oneScene)
         {
             waveWalk(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, [&](uint32_t j) {
-                if (cnt < K)
-                {
-                    lst[cnt * LS] = j;
-                }
+                lst[min(cnt, K) * LS] = j;  // row K is spare: entries past the capacity land there
                 ++cnt;
             });
         }
@@ -552,10 +565,31 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
                 ++cnt;
             });
         }
-        if (!valid)
+        if (valid)
         {
-            continue;
+            c.ncount[t] = cnt;
         }
+    }
+}
+
+// Phase 2 of computeDensities over the neighbour list, plus
+// computeBoundaryDensityContribution; also precomputes p and p / rho^2.
+__global__ __launch_bounds__(256) void kDensity(Ctx c)
+{
+    const uint32_t n = *c.nTotal;
+    const uint32_t K = c.listCap;
+    const size_t LS = c.pCap;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    {
+        const uint32_t s = c.sscene[t];
+        const SceneDev& sc = c.scenes[s];
+        const uint32_t lost = c.states[s].lostEnd;
+        const F4 q = c.s0[t];
+        const float hi = c.shm[t].x;
+        const float R = 2.0F * hi;
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
+        const uint32_t cnt = c.ncount[t];
+        const uint32_t* lst = c.list + t;
         float rho = 0.0F;
         if (cnt <= K)
         {
@@ -638,7 +672,6 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
         rho = rho + rhoB;
         const float p = tait(rho, sc);
         c.s2[t] = F4 {rho, p, p / (rho * rho), hi};
-        c.ncount[t] = cnt;
     }
 }
 
@@ -1648,6 +1681,7 @@ void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
+    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
     hipLaunchKernelGGL(kDensity, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
