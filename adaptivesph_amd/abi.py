@@ -16,7 +16,12 @@ PKG_DIR = Path(__file__).resolve().parent
 LIB_PATH = PKG_DIR / "lib" / "libasph.so"
 
 ASPH_OK = 0
+ASPH_ERR_INVALID = 1
+ASPH_ERR_CAPACITY = 2
+ASPH_ERR_DEVICE = 3
+ASPH_ERR_STATE = 4
 ASPH_FLAG_FIX_CELL_START = 1
+FLAG_KERNEL_MERGED_H = 2
 
 CRITERION = {"velocity": 0, "interface": 1, "vorticity": 2, "zone": 3}
 
@@ -278,6 +283,7 @@ def _extra_sigs():
         "asph_batch_copy_scene": (C.c_int, [vp, u32, u32, vp, vp, vp, vp, vp, vp, vp]),
         "asph_batch_set_launch": (C.c_int, [vp, u32, u32, u32]),
         "asph_batch_memory": (C.c_int, [vp, P(u64), P(u64)]),
+        "asph_batch_step_particles": (C.c_int, [vp, P(u32)]),
         "asph_batch_profile_step": (C.c_int, [vp, P(f32), u32, P(u32)]),
         "asph_profile_stage_name": (C.c_char_p, [u32]),
         "asph_scene_bytes": (u64, [u32, u32, u32, u32, u32]),
@@ -301,7 +307,15 @@ BATCH_GRAPH = 1
 BATCH_ADAPTIVE = 2
 
 
+class AsphError(RuntimeError):
+    """A non-zero status of the C ABI; `status` is the ASPH_ERR_* code."""
+
+    def __init__(self, status: int, message: str):
+        super().__init__(message)
+        self.status = status
+
+
 def check(lib: C.CDLL, rc: int, what: str) -> None:
     if rc != ASPH_OK:
         msg = lib.asph_last_error().decode(errors="replace")
-        raise RuntimeError(f"{what} failed (status {rc}): {msg}")
+        raise AsphError(rc, f"{what} failed (status {rc}): {msg}")

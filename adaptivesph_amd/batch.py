@@ -88,6 +88,11 @@ class Batch:
     def synchronize(self) -> None:
         self._check(self._lib.asph_batch_synchronize(self._h), "synchronize")
 
+    def step_particles(self) -> int:
+        n = C.c_uint32()
+        self._check(self._lib.asph_batch_step_particles(self._h, C.byref(n)), "step particles")
+        return int(n.value)
+
     def count(self, sid: int) -> int:
         n = C.c_uint32()
         self._check(self._lib.asph_batch_scene_count(self._h, int(sid), C.byref(n)), "count")

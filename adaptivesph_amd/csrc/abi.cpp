@@ -392,6 +392,14 @@ int asph_batch_step_timed(asph_batch* batch, float* milliseconds)
     });
 }
 
+int asph_batch_step_particles(asph_batch* batch, uint32_t* count)
+{
+    return guard([&] {
+        need(count, "count");
+        *count = engineOf(batch).lastStepParticles();
+    });
+}
+
 int asph_batch_scene_count(asph_batch* batch, uint32_t scene_id, uint32_t* count)
 {
     return guard([&] {

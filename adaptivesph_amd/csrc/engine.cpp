@@ -177,11 +177,18 @@ void validateParams(const asph_parameters& p)
 
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
-    uint64_t perParticle = 64ULL + 16ULL + 60ULL + 4ULL * listCap;
+    // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
+    // s0/s1/s2 48 + shm 8 + scene id 4 + neighbour count 4, group boxes 4,
+    // neighbour list 4 (listCap + 1), radix histogram and scan tiles < 1.
+    uint64_t perParticle = 64ULL + 16ULL + 64ULL + 4ULL + 4ULL * (listCap + 1ULL) + 1ULL;
     if (adaptive)
     {
-        perParticle += 24ULL + 16ULL + 16ULL + 64ULL + 4ULL;
+        // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
+        // 16, compaction copies 64, merge list capture 4
+        perParticle += 24ULL + 16ULL + 64ULL + 4ULL;
     }
+    // boundary slot: position + psi 16, viscosity 4; cell: count, offset and
+    // boundary offset 12.
     return perParticle * capacity + 20ULL * boundaryCount + 12ULL * (cells + 1ULL);
 }
 
@@ -292,9 +299,11 @@ Engine::Engine(const EngineConfig& cfg)
             hipCheck(hipMemset(c.cand, 0, sizeof(uint32_t) * P), "hipMemset");
         }
         hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        keyBits_ = bitsFor(C - 1);
         sceneBits_ = bitsFor(static_cast<uint64_t>(S));
         scenes_.resize(S);
+        c.nMax = 1;
+        c.cExtent = 2;
+        keyBits_ = 1;
         buildCtx();
     }
     catch (...)
@@ -503,7 +512,8 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     d.bOff = static_cast<uint32_t>(bOff);
     d.bCount = nb;
     d.flags = kSceneActive | (adaptive ? kSceneAdaptive : 0U) |
-              ((p.flags & ASPH_FLAG_FIX_CELL_START) != 0U ? kSceneFixCellStart : 0U);
+              ((p.flags & ASPH_FLAG_FIX_CELL_START) != 0U ? kSceneFixCellStart : 0U) |
+              ((p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? kSceneKernelMergedH : 0U);
     d.tpb = p.threads_per_block;
     if (r != nullptr)
     {
@@ -676,6 +686,33 @@ void Engine::uploadOrder()
         hipCheck(hipMemcpy(cellOrderDev_, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice), "upload");
     }
     hipCheck(hipMemcpy(nOrderDev_, &n, sizeof(uint32_t), hipMemcpyHostToDevice), "upload");
+    updateExtents();
+}
+
+void Engine::updateExtents()
+{
+    uint64_t nMax = 0;
+    uint64_t cExtent = 2;
+    for (const HostScene& hs : scenes_)
+    {
+        if (hs.active)
+        {
+            nMax += hs.dev.pCap;
+            cExtent = std::max<uint64_t>(cExtent, static_cast<uint64_t>(hs.dev.cellBase) + hs.cellRegion);
+        }
+    }
+    nMax = std::max<uint64_t>(nMax, 1);
+    const int bits = bitsFor(cExtent - 1);
+    if (nMax == ctx_.nMax && cExtent == ctx_.cExtent && bits == keyBits_)
+    {
+        return;
+    }
+    // kernels take the context by value, so captured steps are re-captured
+    ctx_.nMax = static_cast<uint32_t>(nMax);
+    ctx_.cExtent = static_cast<uint32_t>(cExtent);
+    keyBits_ = bits;
+    buildCtx();
+    invalidateGraphs();
 }
 
 void Engine::removeScene(uint32_t id)
@@ -813,7 +850,6 @@ void Engine::growCells(uint64_t newCap)
            scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(C) + 1)));
     ctx_.cCap = static_cast<uint32_t>(C);
     cAlloc_.grow(C);
-    keyBits_ = bitsFor(C - 1);
     buildCtx();
     invalidateGraphs();
 }
@@ -909,10 +945,10 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     mark();
     launchKickDriftKey(c, lc, s);
     mark();
-    radixSortU32(c.keysA, c.valsA, c.keysB, c.valsB, c.nTotal, c.pCap, keyBits_, lc.radixBits, hist_, digitTotals_,
+    radixSortU32(c.keysA, c.valsA, c.keysB, c.valsB, c.nTotal, c.nMax, keyBits_, lc.radixBits, hist_, digitTotals_,
                  s);
     mark();
-    exclusiveScanU32(c.cellCount, c.cellOff, nullptr, c.cCap, tileSums_, scanTotal_, true, s);
+    exclusiveScanU32(c.cellCount, c.cellOff, nullptr, c.cExtent, tileSums_, scanTotal_, true, s);
     mark();
     launchSceneRanges(c, s);
     mark();
@@ -926,7 +962,7 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     {
         launchMergeCriteria(c, lc, s);
         mark();
-        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.pCap, 32 + sceneBits_, hist_, digitTotals_,
+        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
                      s);
         mark();
         launchMergeSelect(c, s);
@@ -938,13 +974,13 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
         launchResolvePairs(c, lc, s);
         launchMergeApply(c, lc, s);
         mark();
-        exclusiveScanU32(c.removal, c.rprefix, c.nTotal, c.pCap, tileSums_, scanTotal_, false, s);
+        exclusiveScanU32(c.removal, c.rprefix, c.nTotal, c.nMax, tileSums_, scanTotal_, false, s);
         launchCompact(c, lc, s);
         launchAfterMerge(c, s);
         launchCopyBack(c, lc, s);
         mark();
         launchSplitCriteria(c, lc, s);
-        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.pCap, 32 + sceneBits_, hist_, digitTotals_,
+        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
                      s);
         launchSplitSelect(c, s);
         launchSplitApply(c, lc, s);
@@ -1039,6 +1075,14 @@ void Engine::step(uint32_t steps)
 void Engine::synchronize() { hipCheck(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
 
 uint32_t Engine::count(uint32_t id) { return readState(id).count; }
+
+uint32_t Engine::lastStepParticles()
+{
+    synchronize();
+    uint32_t n = 0;
+    hipCheck(hipMemcpy(&n, ctx_.nTotal, sizeof(n), hipMemcpyDeviceToHost), "download");
+    return n;
+}
 
 uint32_t Engine::boundaryCount(uint32_t id) const { return scenes_.at(id).dev.bCount; }
 

@@ -101,6 +101,8 @@ public:
     void synchronize();
 
     uint32_t count(uint32_t id);  // synchronous
+    // Particles of all scenes at the start of the most recent step (synchronous).
+    uint32_t lastStepParticles();
     uint32_t boundaryCount(uint32_t id) const;
     uint32_t capacity(uint32_t id) const;
     void copyFluid(uint32_t id, uint32_t mask, asph_vec4* pos, asph_vec4* vel, asph_vec4* acc, float* rho,
@@ -147,6 +149,7 @@ private:
     void buildCtx();
     void uploadScene(uint32_t id);
     void uploadOrder();
+    void updateExtents();
     void uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32_t nb);
     void enqueueStep(bool refine, std::vector<hipEvent_t>* marks);
     bool anyRefineNext() const;

@@ -48,7 +48,12 @@ struct Ctx
     float* rad;
     float* h;
     float* m;
-    uint32_t pCap;  // pool capacity (particles)
+    uint32_t pCap;  // pool capacity (particles); stride of the neighbour list rows
+    // Launch extents of the active scenes (host-set when scenes are added or
+    // removed): nMax bounds the particles of a step (sum of the active
+    // scenes' capacities), cExtent the cells in use (highest cell slot + 1).
+    uint32_t nMax;
+    uint32_t cExtent;
 
     // neighbour grid
     uint32_t* keysA;

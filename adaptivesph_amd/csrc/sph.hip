@@ -1304,7 +1304,8 @@ __global__ __launch_bounds__(256) void kMergeApply(Ctx c)
         c.acc[a] = na;
         c.m[a] = M;
         c.rad[a] = sc.r0 * detCbrt(M / sc.m0);
-        c.h[a] = detCbrt((kMergeCoef * M) / denom);
+        const float coef = (sc.flags & kSceneKernelMergedH) != 0U ? kWCoef : kMergeCoef;
+        c.h[a] = detCbrt((coef * M) / denom);
         c.rho[a] = 0.0F;
     }
 }
@@ -1669,32 +1670,32 @@ void launchPrep(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kPrep, dim3(1)
 
 void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kKickDriftKey, dim3(gridFor(c.pCap, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
+    hipLaunchKernelGGL(kKickDriftKey, dim3(gridFor(c.nMax, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
 }
 
 void launchSceneRanges(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kSceneRanges, dim3(1), dim3(256), 0, s, c); }
 
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kGather, dim3(gridFor(c.pCap, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
+    hipLaunchKernelGGL(kGather, dim3(gridFor(c.nMax, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
 }
 
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(kDensity, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kDensity, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kForce, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kForce, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchMergeCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kMergeCriteria, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kMergeCriteria, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
@@ -1702,25 +1703,25 @@ void launchMergeSelect(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kMergeS
 
 void launchMergeCandidates(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kMergeCandidates, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kMergeCandidates, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchResolvePairs(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kResolvePairs, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kResolvePairs, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchMergeApply(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kMergeApply, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kMergeApply, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchCompact(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kCompact, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kCompact, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
@@ -1728,13 +1729,13 @@ void launchAfterMerge(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kAfterMe
 
 void launchCopyBack(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kCopyBack, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kCopyBack, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
 void launchSplitCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kSplitCriteria, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kSplitCriteria, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 
@@ -1742,7 +1743,7 @@ void launchSplitSelect(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kSplitS
 
 void launchSplitApply(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kSplitApply, dim3(gridFor(c.pCap, 256)), dim3(256), 0, s, c);
+    hipLaunchKernelGGL(kSplitApply, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
     (void)lc;
 }
 

@@ -169,16 +169,29 @@ def sloshing_tank(n=(125, 50, 80), dx: float = 0.008, box=(1.0, 0.6, 0.64), seed
                  {"n": tuple(n), "dx": dx, "box": tuple(box), "seed": seed})
 
 
-def cfg3(boundary: str = "library") -> Scene:
+def kernel_merged_h(s: Scene) -> Scene:
+    """Switches the scene to the kernel-normalised merged smoothing length
+    (abi.FLAG_KERNEL_MERGED_H). With the reference's coefficient every merge
+    multiplies h by about 1.8-2, and split/merge cycles under frequent
+    refinement grow h without bound (over 100 h0 within 100 steps of a
+    sloshing tank refining every step), until neighbour searches span the
+    whole tank."""
+    s.params.flags |= abi.FLAG_KERNEL_MERGED_H
+    s.meta["merged_h"] = "kernel"
+    return s
+
+
+def cfg3(boundary: str = "library", merged_h: str = "kernel") -> Scene:
     """Adaptive sloshing tank: 500,000 particles at dx = 0.008 m (125 x 50 x 80 in a
     1.0 x 0.6 x 0.64 m tank), refinement zone r = 0.1 m on a circle of radius
     0.2 m (period 2 s), refinement every step, cap 4M, 2000 steps."""
-    return sloshing_tank(boundary=boundary, name="cfg3")
+    s = sloshing_tank(boundary=boundary, name="cfg3")
+    return kernel_merged_h(s) if merged_h == "kernel" else s
 
 
 def random_scene(seed: int, n_min: int = 5_000, n_max: int = 200_000, box=(1.0, 0.6, 0.5),
                  adaptive: bool | None = None, kind: str | None = None, boundary: str = "library",
-                 steps: int = 500) -> Scene:
+                 steps: int = 500, merged_h: str = "reference") -> Scene:
     """One scene of the batched / served workloads: log-uniform size, dam break
     (fill height U[0.2, 0.6] of the box) or sloshing tank, refinement on or off."""
     rng = np.random.default_rng(seed)
@@ -207,9 +220,12 @@ def random_scene(seed: int, n_min: int = 5_000, n_max: int = 200_000, box=(1.0, 
         if adaptive:
             sc.refinement.zone.radius = 0.1 * box[0]
     sc.meta.update({"kind": kind, "fill": fill, "target": target})
+    if merged_h == "kernel" and adaptive:
+        kernel_merged_h(sc)
     return sc
 
 
-def cfg4_scenes(count: int = 256, boundary: str = "library", steps: int = 500) -> list[Scene]:
-    """256 independent scenes, per-scene seed 1000 + i (about 13M particles)."""
-    return [random_scene(1000 + i, boundary=boundary, steps=steps) for i in range(count)]
+def cfg4_scenes(count: int = 256, boundary: str = "library", steps: int = 500,
+                merged_h: str = "kernel") -> list[Scene]:
+    """256 independent scenes, per-scene seed 1000 + i (about 13-14M particles)."""
+    return [random_scene(1000 + i, boundary=boundary, steps=steps, merged_h=merged_h) for i in range(count)]

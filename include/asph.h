@@ -43,6 +43,13 @@ typedef struct asph_domain {
 #define ASPH_FLAG_FIX_CELL_START 1u /* build cell ranges without the reference's
                                       slot-0 omission (NeighborKernels.cu:27-36);
                                       default off = reference behaviour */
+#define ASPH_FLAG_KERNEL_MERGED_H 2u /* merged smoothing length with the Wendland
+                                       normalisation 21/(16 pi), so that a
+                                       coincident equal pair keeps its h,
+                                       instead of the reference's 16 pi/21
+                                       (AdaptiveAlgorithm.cu:62-66), which
+                                       grows h by (16 pi/21)^(2/3) = 1.79 per
+                                       merge; default off = reference */
 
 /* Simulation::Parameters (Simulation.cuh:64-109) + extensions. */
 typedef struct asph_parameters {

@@ -58,6 +58,10 @@ int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, a
    and graph replay on/off. */
 int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t radix_bits, uint32_t use_graph);
 
+/* Particles of all scenes stepped by the most recent step (their count at
+   the start of that step, before refinement). Synchronous. */
+int asph_batch_step_particles(asph_batch* batch, uint32_t* count);
+
 /* Device bytes allocated by the batch and particle slots in use. */
 int asph_batch_memory(asph_batch* batch, uint64_t* allocated, uint64_t* particles_used);
 

@@ -68,7 +68,7 @@ def load() -> C.CDLL:
         "oracle_mix64": (C.c_uint64, [C.c_uint64]),
         "oracle_boundary_generate": (u32, [P(abi.Domain), f32, f32, f32, vp]),
         "oracle_split_one": (C.c_int, [vp, u32, u32]),
-        "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32]),
+        "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32, u32]),
         "oracle_top_list": (u32, [vp, u32, C.c_int, f32, vp]),
         "oracle_cell_ranges": (None, [vp, u32, u32, u32, vp, vp]),
     }
@@ -247,11 +247,11 @@ def cell_ranges(sorted_keys: np.ndarray, cells: int, fix: bool = False) -> tuple
     return s, e
 
 
-def merged_h(pa, pb, ma, mb, ha, hb, pm, M) -> float:
+def merged_h(pa, pb, ma, mb, ha, hb, pm, M, flags: int = 0) -> float:
     a = np.asarray(pa, np.float32)
     b = np.asarray(pb, np.float32)
     m = np.asarray(pm, np.float32)
-    return float(load().oracle_merged_h(a.ctypes.data, b.ctypes.data, ma, mb, ha, hb, m.ctypes.data, M))
+    return float(load().oracle_merged_h(a.ctypes.data, b.ctypes.data, ma, mb, ha, hb, m.ctypes.data, M, flags))
 
 
 def wendland(d, h, precision=64):

@@ -973,7 +973,9 @@ public:
         const T wa = wendland(da, ha);
         const T wb = wendland(db, hb);
         const T denom = (ma * wa) + (mb * wb);
-        return Const<T>::cbrt((Const<T>::mergeCoef * M) / denom);
+        // ASPH_FLAG_KERNEL_MERGED_H: the Wendland normalisation (WendlandKernel.cuh:11)
+        const T coef = (p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? Const<T>::wcoef : Const<T>::mergeCoef;
+        return Const<T>::cbrt((coef * M) / denom);
     }
 
     void performAdaptiveRefinement(double t)
@@ -1790,9 +1792,10 @@ int oracle_split_one(oracle_sim* s, uint32_t parent, uint32_t base)
 
 // Merged smoothing length of two particles (calculateMergedSmoothingLength).
 float oracle_merged_h(const float* pa, const float* pb, float ma, float mb, float ha, float hb, const float* pm,
-                      float M)
+                      float M, uint32_t flags)
 {
     oracle::Sim<float> tmp;
+    tmp.p.flags = flags;
     const oracle::V4<float> a {pa[0], pa[1], pa[2], pa[3]};
     const oracle::V4<float> b {pb[0], pb[1], pb[2], pb[3]};
     const oracle::V4<float> m {pm[0], pm[1], pm[2], pm[3]};

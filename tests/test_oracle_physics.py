@@ -287,6 +287,22 @@ def test_merged_smoothing_length_closed_form():
     db = abs(pb[0] - pm[0])
     want = (16 * math.pi / 21 * M / (ma * oracle.wendland(da, ha) + mb * oracle.wendland(db, hb))) ** (1 / 3)
     assert got == pytest.approx(want, rel=1e-5)
+    kern = oracle.merged_h(pa, pb, ma, mb, ha, hb, pm, M, abi.FLAG_KERNEL_MERGED_H)
+    want = (21 / (16 * math.pi) * M / (ma * oracle.wendland(da, ha) + mb * oracle.wendland(db, hb))) ** (1 / 3)
+    assert kern == pytest.approx(want, rel=1e-5)
+
+
+def test_merged_smoothing_length_coincident_pair():
+    # two equal particles at one point: W = 21/(16 pi h^3) for both, so the
+    # kernel-normalised form keeps h exactly and the reference form scales it
+    # by (16 pi/21)^(2/3)
+    p = [0.25, 0.5, 0.75, 0.0]
+    m, h = 1e-3, 0.0125
+    ref = oracle.merged_h(p, p, m, m, h, h, p, 2 * m)
+    kern = oracle.merged_h(p, p, m, m, h, h, p, 2 * m, abi.FLAG_KERNEL_MERGED_H)
+    assert kern == pytest.approx(h, rel=2e-7)
+    assert ref == pytest.approx(h * (16 * math.pi / 21) ** (2 / 3), rel=1e-6)
+    assert (16 * math.pi / 21) ** (2 / 3) == pytest.approx(1.789371, rel=1e-6)
 
 
 def test_top_list_ties_and_sentinels():
