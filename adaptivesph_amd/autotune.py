@@ -1,38 +1,58 @@
 """Launch-shape autotuner of a batch.
 
-Searches, in a fixed order, the block size of the streaming kernels and the
-digit width of the cell-key radix sort, timing each candidate on the batch's
-own scenes with HIP events, and keeps the fastest. Every candidate computes
-bit-identical results (the launch shape changes only how work is split), so
-the tuning steps are ordinary steps of the simulation: they advance the
-scenes, and count as warm-up where a benchmark follows.
+Three launch parameters shape a step, each acting on its own stages:
+
+* block_stream   threads per block of the streaming kernels (kick/drift/key,
+                 gather);
+* radix_bits     digit width of the cell-key radix sort (cell sort);
+* block_neighbor threads per block of the neighbour kernels (density, forces).
+
+The tuner measures, in a fixed order, each candidate value on the stages it
+acts on (HIP events between the stages of eager steps of the batch's own
+scenes), normalised by the particles of that step, and keeps the fastest
+(coordinate search: one parameter at a time, the others at their current
+choice). The scenes evolve while the tuner steps them (adaptive scenes change
+their particle counts), so every candidate is measured in `rounds` interleaved
+rounds and judged by its median. Every launch shape gives bit-identical
+results, so the tuning steps are ordinary steps of the simulation: they
+advance the scenes, and count as warm-up where a benchmark follows.
 """
 from __future__ import annotations
 
 import numpy as np
 
-BLOCK_STREAM = (128, 256, 512, 1024)
-RADIX_BITS = (4, 6, 7, 8)
+SPACE = {
+    "block_stream": ((128, 256, 512, 1024), ("kick_drift_key", "gather")),
+    "radix_bits": ((4, 6, 7, 8), ("cell_sort",)),
+    "block_neighbor": ((64, 128, 256), ("density", "force")),
+}
+DEFAULT = {"block_stream": 256, "radix_bits": 8, "block_neighbor": 256}
 
 
-def candidates(block_stream=BLOCK_STREAM, radix_bits=RADIX_BITS):
-    return [(b, r) for r in radix_bits for b in block_stream]
+def _apply(batch, shape: dict, graph: bool) -> None:
+    batch.set_launch(shape["block_stream"], shape["radix_bits"], graph, block_neighbor=shape["block_neighbor"])
 
 
-def autotune(batch, steps: int = 4, graph: bool = True, cands=None) -> dict:
-    """Times `steps` steps per candidate (after one untimed step that captures
-    the candidate's graph) and sets the candidate with the lowest median step
-    time. Ties keep the earlier candidate. Returns the choice and the table."""
-    table = []
-    best = None
-    for bs, rb in cands or candidates():
-        batch.set_launch(bs, rb, graph)
-        batch.step(1)
-        batch.synchronize()
-        ms = float(np.median([batch.step_timed() for _ in range(steps)]))
-        table.append({"block_stream": bs, "radix_bits": rb, "ms": round(ms, 4)})
-        if best is None or ms < best[2]:
-            best = (bs, rb, ms)
-    batch.set_launch(best[0], best[1], graph)
-    return {"block_stream": best[0], "radix_bits": best[1], "ms": round(best[2], 4), "table": table,
-            "steps_used": len(table) * (steps + 1)}
+def autotune(batch, rounds: int = 3, graph: bool = True, space: dict | None = None) -> dict:
+    """Chooses and sets the launch shape; returns it with the measured
+    per-particle stage times (ns per particle) of every candidate."""
+    space = space or SPACE
+    shape = dict(DEFAULT)
+    table = {}
+    steps = 0
+    for param, (values, stages) in space.items():
+        samples = {v: [] for v in values}
+        for _ in range(rounds):
+            for v in values:
+                trial = dict(shape, **{param: v})
+                _apply(batch, trial, graph)
+                ms = batch.profile_step()
+                n = max(batch.step_particles(), 1)
+                steps += 1
+                samples[v].append(sum(ms.get(s, 0.0) for s in stages) * 1e6 / n)
+        med = {v: float(np.median(samples[v])) for v in values}
+        best = min(values, key=lambda v: (med[v], values.index(v)))
+        shape[param] = best
+        table[param] = {str(v): round(med[v], 4) for v in values}
+    _apply(batch, shape, graph)
+    return {**shape, "ns_per_particle": table, "steps_used": steps}

@@ -113,8 +113,10 @@ class Batch:
                     "copy scene")
         return out
 
-    def set_launch(self, block_stream: int = 256, radix_bits: int = 8, graph: bool = True) -> None:
-        self._check(self._lib.asph_batch_set_launch(self._h, block_stream, radix_bits, int(graph)), "set launch")
+    def set_launch(self, block_stream: int = 256, radix_bits: int = 8, graph: bool = True,
+                   block_neighbor: int = 256) -> None:
+        self._check(self._lib.asph_batch_set_launch(self._h, block_stream, block_neighbor, radix_bits, int(graph)),
+                    "set launch")
 
     def memory(self) -> tuple[int, int]:
         a = C.c_uint64()

@@ -421,7 +421,8 @@ int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, a
     });
 }
 
-int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t radix_bits, uint32_t use_graph)
+int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t block_neighbor, uint32_t radix_bits,
+                          uint32_t use_graph)
 {
     return guard([&] {
         asph::Engine& e = engineOf(batch);
@@ -429,12 +430,17 @@ int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t rad
         {
             throw asph::Error(ASPH_ERR_INVALID, "block_stream must be a multiple of 64 in [64, 1024]");
         }
+        if (block_neighbor != 64 && block_neighbor != 128 && block_neighbor != 256)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "block_neighbor must be 64, 128 or 256");
+        }
         if (radix_bits != 4 && radix_bits != 6 && radix_bits != 7 && radix_bits != 8)
         {
             throw asph::Error(ASPH_ERR_INVALID, "radix_bits must be 4, 6, 7 or 8");
         }
         asph::LaunchCfg lc = e.config().launch;
         lc.blockStream = block_stream;
+        lc.blockNeighbor = block_neighbor;
         lc.radixBits = static_cast<int>(radix_bits);
         e.setLaunch(lc);
         e.setUseGraph(use_graph != 0U);

@@ -108,7 +108,7 @@ struct Ctx
 struct LaunchCfg
 {
     uint32_t blockStream = 256;  // streaming kernels
-    uint32_t blockNeighbor = 256;  // neighbour kernels
+    uint32_t blockNeighbor = 256;  // neighbour kernels (64, 128 or 256)
     int radixBits = 8;
 };
 

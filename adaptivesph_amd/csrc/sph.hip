@@ -1682,15 +1682,15 @@ void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
-    hipLaunchKernelGGL(kDensity, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
-    (void)lc;
+    const uint32_t b = lc.blockNeighbor;
+    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    hipLaunchKernelGGL(kDensity, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
 }
 
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    hipLaunchKernelGGL(kForce, dim3(gridFor(c.nMax, 256)), dim3(256), 0, s, c);
-    (void)lc;
+    const uint32_t b = lc.blockNeighbor;
+    hipLaunchKernelGGL(kForce, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
 }
 
 void launchMergeCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s)

@@ -191,14 +191,14 @@ def run_batch(args) -> dict:
     tuning = Batch.for_scenes(scenes, graph=True)
     for s in scenes:
         tuning.add(s)
-    tune = autotune(tuning, steps=3)
+    tune = autotune(tuning)
     tuning.close()
-    log("autotune:", json.dumps({k: v for k, v in tune.items() if k != "table"}))
+    log("autotune:", json.dumps(tune))
 
     batch = Batch.for_scenes(scenes, graph=True)
     for s in scenes:
         batch.add(s)
-    batch.set_launch(tune["block_stream"], tune["radix_bits"], True)
+    batch.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
     batch.step(args.warmup)
     batch.synchronize()
     particles = 0
@@ -222,7 +222,7 @@ def run_batch(args) -> dict:
     for k, s in enumerate(scenes):
         b = Batch.for_scenes([s], graph=True)
         b.add(s)
-        b.set_launch(tune["block_stream"], tune["radix_bits"], True)
+        b.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
         b.step(args.warmup)
         b.synchronize()
         t2 = time.perf_counter()
@@ -239,13 +239,13 @@ def run_batch(args) -> dict:
                     elapsed / args.steps * 1e3,
                     {"workload": workload, "scenes": len(scenes), "particles": total0,
                      "adaptive_scenes": sum(s.adaptive for s in scenes), "merged_h": args.merged_h, "graph": True,
-                     "launch": {"block_stream": tune["block_stream"], "radix_bits": tune["radix_bits"]}})
+                     "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor")}})
     out["particle_updates_per_s"] = value
     out["step_ms_p50"] = float(np.percentile(lat, 50))
     out["step_ms_p99"] = float(np.percentile(lat, 99))
     out["sequential"] = {"particle_updates_per_s": seq_value, "scene_steps_per_s": len(scenes) * args.steps /
                          seq_elapsed, "elapsed_s": seq_elapsed, "speedup": value / seq_value}
-    out["autotune"] = tune["table"]
+    out["autotune"] = tune["ns_per_particle"]
     out["roofline"] = roofline_of(stage_ms, stage_particles, workload)
     out["cpu_baseline"] = None
     out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}

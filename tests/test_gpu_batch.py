@@ -82,15 +82,15 @@ def test_continuous_admission_and_removal():
 def test_launch_shapes_do_not_change_results():
     scenes = _mix()[:3]
     ref = None
-    for block, radix in ((256, 8), (128, 6), (512, 4), (64, 7)):
+    for block, radix, nb in ((256, 8, 256), (128, 6, 64), (512, 4, 128), (64, 7, 256), (1024, 8, 64)):
         b = Batch.for_scenes(scenes, graph=True)
         ids = [b.add(s) for s in scenes]
-        b.set_launch(block, radix, True)
+        b.set_launch(block, radix, True, block_neighbor=nb)
         b.step(6)
         out = [b.fluid(i) for i in ids]
         if ref is None:
             ref = out
         else:
             for s, x, y in zip(scenes, out, ref):
-                assert_bitexact(x, y, f"{s.name} block={block} radix={radix}")
+                assert_bitexact(x, y, f"{s.name} block={block} radix={radix} neighbour block={nb}")
         b.close()
