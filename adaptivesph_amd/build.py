@@ -62,5 +62,24 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
+def build_cpp_test(verbose: bool = True) -> Path:
+    """The C++ interface test program (tests/cpp/facade_test.cpp), compiled by
+    the host compiler against include/ and linked to the library."""
+    root = PKG.parent
+    out = LIB.parent / "facade_test"
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(root / "include"),
+           str(root / "tests" / "cpp" / "facade_test.cpp"), "-L", str(LIB.parent), "-lasph",
+           "-Wl,-rpath,$ORIGIN", "-o", str(out)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ interface test build failed:\n{r.stdout}\n{r.stderr}")
+    if r.stderr.strip():
+        print(f"[build] facade_test: {r.stderr[:4000]}", file=sys.stderr)
+    if verbose:
+        print(f"[build] {out}")
+    return out
+
+
 if __name__ == "__main__":
     build()
+    build_cpp_test()

@@ -100,13 +100,16 @@ def physics(box_min, box_max, dx: float, gamma: float = 7.0, nu: float = 1e-6, h
     return p, dt
 
 
+def boundary_spacing(params: abi.Parameters) -> float:
+    """Wall particle spacing 2 r (App.cpp:142) in FP32."""
+    return float(np.float32(params.base_particle_radius) * np.float32(2.0))
+
+
 def generate_boundary(params: abi.Parameters, generator: str = "library") -> np.ndarray:
     """StaticBoundaryDomain::generate(domain, 2 r, rho0, h) (App.cpp:143-148)."""
-    spacing = float(np.float32(params.base_particle_radius) * np.float32(2.0))
-    if generator == "oracle":  # test-only path, used when checking the product generator
-        from oracle import oracle as _oracle  # noqa: PLC0415
-
-        return _oracle.boundary_generate(params.domain, spacing, params.rest_density, params.base_smoothing_radius)
+    if generator != "library":
+        raise ValueError(f"unknown boundary generator {generator!r}")
+    spacing = boundary_spacing(params)
     lib = abi.load_library()
     n = C.c_uint32(0)
     abi.check(lib, lib.asph_boundary_generate(C.byref(params.domain), spacing, params.rest_density,

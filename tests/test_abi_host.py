@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from adaptivesph_amd import abi, scene
+from oracle import oracle
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -120,7 +121,8 @@ def test_grid_cells_of_the_workloads():
 def test_boundary_generator_matches_oracle(which, count):
     s = getattr(scene, which)(boundary="")
     lib_b = scene.generate_boundary(s.params, "library")
-    ora_b = scene.generate_boundary(s.params, "oracle")
+    ora_b = oracle.boundary_generate(s.params.domain, scene.boundary_spacing(s.params), s.params.rest_density,
+                                     s.params.base_smoothing_radius)
     assert len(lib_b) == count
     assert np.array_equal(lib_b.view(np.uint8), ora_b.view(np.uint8))
 

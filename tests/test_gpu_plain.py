@@ -171,3 +171,58 @@ def test_invalid_arguments_raise():
     pos[0, 3] = 1.0
     with pytest.raises(RuntimeError, match="w == 0"):
         Simulation(s.params, pos, s.boundary)
+
+
+def test_cfg2_first_step_bitexact_and_fp64_tolerance():
+    """The 1,048,576-particle dam break: keys, sorted order, cell ranges,
+    neighbour sets and state after one step equal the FP32 oracle's bits;
+    density and acceleration agree with the FP64 oracle to 1e-4."""
+    s = scene.cfg2()
+    sim, ref = _pair(s)
+    sim.update(s.dt)
+    ref.update(s.dt)
+    g, go = sim.grid(), ref.grid()
+    for k in ("keys", "indices", "cell_start", "cell_end"):
+        assert np.array_equal(g[k], go[k]), k
+    c, fp = sim.neighbor_sets()
+    co, fpo = ref.neighbor_sets()
+    assert np.array_equal(c, co) and np.array_equal(fp, fpo)
+    assert c.max() == 80
+    got = sim.fluid()
+    assert_bitexact(got, ref.fluid(), "cfg2 step 1")
+    ref.close()
+    r64 = oracle.OracleSim.from_scene(s, precision=64)
+    r64.update(s.dt)
+    f64 = r64.fluid64()
+    rho_rel = np.abs(got["densities"].astype(np.float64) - f64["densities"]) / f64["densities"]
+    assert rho_rel.max() <= 1e-4
+    a64 = f64["accelerations"][:, :3]
+    da = np.linalg.norm(got["accelerations"][:, :3].astype(np.float64) - a64, axis=1)
+    assert da.max() <= 1e-4 * np.linalg.norm(a64, axis=1).mean()
+    # size-independent invariants of the step: unit masses unchanged, every
+    # particle inside the box, densities positive
+    assert (got["masses"] == np.float32(s.params.base_particle_mass)).all()
+    lo = np.array([s.params.domain.min.x, s.params.domain.min.y, s.params.domain.min.z], np.float32)
+    hi = np.array([s.params.domain.max.x, s.params.domain.max.y, s.params.domain.max.z], np.float32)
+    assert ((got["positions"][:, :3] >= lo) & (got["positions"][:, :3] <= hi)).all()
+    assert (got["densities"] > 0).all()
+
+
+def test_cfg1_fp64_drift_within_stated_bounds():
+    """Trajectory drift against the FP64 oracle (the GPU equals the FP32
+    oracle bit for bit, so this is FP32's own rounding noise): first step
+    within 1e-4, and the drift bounds quoted with the results."""
+    s = scene.cfg1()
+    sim = Simulation.from_scene(s)
+    r64 = oracle.OracleSim.from_scene(s, precision=64)
+    dx = s.meta["dx"]
+    bounds = {1: (1e-5, 1e-5), 10: (1e-4, 1e-4), 50: (2e-3, 1e-3), 100: (2e-2, 1e-2)}
+    for step in range(1, 101):
+        sim.update(s.dt)
+        r64.update(s.dt)
+        if step in bounds:
+            a, b = sim.fluid(), r64.fluid64()
+            d = np.linalg.norm(a["positions"][:, :3].astype(np.float64) - b["positions"][:, :3], axis=1) / dx
+            rho = np.abs(a["densities"].astype(np.float64) - b["densities"]) / b["densities"]
+            assert d.max() <= bounds[step][0], (step, d.max())
+            assert rho.max() <= bounds[step][1], (step, rho.max())
