@@ -24,7 +24,7 @@ import numpy as np
 SPACE = {
     "block_stream": ((128, 256, 512, 1024), ("kick_drift_key", "gather")),
     "radix_bits": ((4, 6, 7, 8), ("cell_sort",)),
-    "block_neighbor": ((64, 128, 256), ("density", "force")),
+    "block_neighbor": ((64, 128, 256), ("neighbors", "density", "force")),
 }
 DEFAULT = {"block_stream": 256, "radix_bits": 8, "block_neighbor": 256}
 

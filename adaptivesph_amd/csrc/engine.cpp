@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <numeric>
 
@@ -175,12 +176,19 @@ void validateParams(const asph_parameters& p)
     }
 }
 
+uint32_t listCapacityRounded(uint32_t listCap)
+{
+    const uint32_t k = listCap == 0U ? 4U : listCap;
+    return ((k + 3U) / 4U) * 4U;
+}
+
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
-    // s0/s1/s2 48 + shm 8 + scene id 4 + neighbour count 4, group boxes 4,
-    // neighbour list 4 (listCap + 1), radix histogram and scan tiles < 1.
-    uint64_t perParticle = 64ULL + 16ULL + 64ULL + 4ULL + 4ULL * (listCap + 1ULL) + 1ULL;
+    // s0/s1/s2 48 + shm 16 + scene id 4 + neighbour count 4, group boxes 4,
+    // neighbour list 4 per entry (capacity rounded up to whole 4-entry
+    // chunks), overflow queue 4, radix histogram and scan tiles < 1.
+    uint64_t perParticle = 64ULL + 16ULL + 72ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 4ULL + 1ULL;
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
@@ -188,8 +196,8 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
         perParticle += 24ULL + 16ULL + 64ULL + 4ULL;
     }
     // boundary slot: position + psi 16, viscosity 4; cell: count, offset and
-    // boundary offset 12.
-    return perParticle * capacity + 20ULL * boundaryCount + 12ULL * (cells + 1ULL);
+    // boundary offset 12, boundary box 32.
+    return perParticle * capacity + 20ULL * boundaryCount + 44ULL * (cells + 1ULL);
 }
 
 // --------------------------------------------------------------- engine --
@@ -247,17 +255,22 @@ Engine::Engine(const EngineConfig& cfg)
         c.cCap = C;
         c.s0 = alloc<F4>(P);
         c.s1 = alloc<F4>(P);
-        c.shm = alloc<float2>(P);
+        c.shm = alloc<F4>(P);
         c.s2 = alloc<F4>(P);
         c.gbox0 = alloc<F4>(P / kGroup + 1);
         c.gbox1 = alloc<F4>(P / kGroup + 1);
         c.sscene = alloc<uint32_t>(P);
         c.ncount = alloc<uint32_t>(P);
+        c.overflowList = alloc<uint32_t>(P);
+        c.overflowCount = alloc<uint32_t>(1);
+        cfg_.listCap = listCapacityRounded(cfg_.listCap);
         c.listCap = cfg_.listCap;
-        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap + 1U) * P);  // + spare row
+        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap) * P);
         c.bs0 = alloc<F4>(B);
         c.bvisc = alloc<float>(B);
         c.bcellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);
+        c.bbox0 = alloc<F4>(static_cast<size_t>(C) + 1);
+        c.bbox1 = alloc<F4>(static_cast<size_t>(C) + 1);
         if (cfg_.adaptive)
         {
             c.ckeysA = alloc<uint64_t>(P);
@@ -658,6 +671,24 @@ void Engine::uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32
     }
     hipCheck(hipMemcpy(ctx_.bcellOff + d.cellBase, off.data(), sizeof(uint32_t) * (cells + 1), hipMemcpyHostToDevice),
              "upload");
+    // per-cell boxes of the boundary particles (all of the cell's particles,
+    // so the box also covers slots the slot-0 omission hides)
+    const float inf = std::numeric_limits<float>::infinity();
+    std::vector<F4> lo(cells + 1, F4 {inf, inf, inf, 0.0F});
+    std::vector<F4> hi(cells + 1, F4 {-inf, -inf, -inf, 0.0F});
+    for (uint32_t k = 0; k < nb; ++k)
+    {
+        const uint32_t cell = static_cast<uint32_t>(hs.boundaryKeysSorted[k]);
+        const F4& p = s0[k];
+        lo[cell].x = std::min(lo[cell].x, p.x);
+        lo[cell].y = std::min(lo[cell].y, p.y);
+        lo[cell].z = std::min(lo[cell].z, p.z);
+        hi[cell].x = std::max(hi[cell].x, p.x);
+        hi[cell].y = std::max(hi[cell].y, p.y);
+        hi[cell].z = std::max(hi[cell].z, p.z);
+    }
+    hipCheck(hipMemcpy(ctx_.bbox0 + d.cellBase, lo.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
+    hipCheck(hipMemcpy(ctx_.bbox1 + d.cellBase, hi.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
 }
 
 void Engine::uploadScene(uint32_t id)
@@ -846,6 +877,8 @@ void Engine::growCells(uint64_t newCap)
     regrow(ctx_.cellCount, oldC, C);
     regrow(ctx_.cellOff, oldC + 1, C + 1);
     regrow(ctx_.bcellOff, oldC + 1, C + 1);
+    regrow(ctx_.bbox0, oldC + 1, C + 1);
+    regrow(ctx_.bbox1, oldC + 1, C + 1);
     regrow(tileSums_, scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(oldC) + 1)),
            scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(C) + 1)));
     ctx_.cCap = static_cast<uint32_t>(C);
@@ -920,10 +953,9 @@ void Engine::advanceFrames()
     }
 }
 
-const char* const kStageNames[] = {"prep",          "kick_drift_key",  "cell_sort",     "cell_scan",
-                                   "scene_ranges",  "gather",          "density",       "force",
-                                   "merge_criteria", "criterion_sort", "merge",         "compaction",
-                                   "split",         "kick_refined"};
+const char* const kStageNames[] = {"prep",           "kick_drift_key", "cell_sort",  "cell_scan", "scene_ranges",
+                                   "gather",         "neighbors",      "density",    "force",     "merge_criteria",
+                                   "criterion_sort", "merge",          "compaction", "split",     "kick_refined"};
 constexpr uint32_t kStageCount = sizeof(kStageNames) / sizeof(kStageNames[0]);
 
 void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
@@ -953,6 +985,8 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     launchSceneRanges(c, s);
     mark();
     launchGather(c, lc, s);
+    mark();
+    launchNeighbors(c, lc, s);
     mark();
     launchDensity(c, lc, s);
     mark();

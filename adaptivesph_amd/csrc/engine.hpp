@@ -65,7 +65,7 @@ struct EngineConfig
     uint32_t boundaryCapacity = 0;  // pool of boundary particle slots
     uint32_t cellCapacity = 0;      // pool of grid cells
     uint32_t sceneSlots = 1;        // scene table size
-    uint32_t listCap = 128;         // neighbour-list entries per particle
+    uint32_t listCap = 128;         // neighbour-list entries per particle (rounded up to 4s)
     bool adaptive = false;          // allocate refinement scratch
     bool useGraph = false;          // capture and replay one step as a hipGraph
     LaunchCfg launch;
@@ -76,6 +76,8 @@ struct EngineConfig
 const char* stageName(uint32_t i);
 
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap);
+// Neighbour-list capacity actually used: a whole number of 4-entry chunks.
+uint32_t listCapacityRounded(uint32_t listCap);
 
 class Engine
 {

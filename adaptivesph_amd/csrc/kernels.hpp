@@ -69,12 +69,16 @@ struct Ctx
     // sorted shadow copies
     F4* s0;      // (x, y, z, (2h)^2)
     F4* s1;      // (vx, vy, vz, m)
-    float2* shm;  // (h, m)
-    F4* s2;      // (rho, p, p / rho^2, h)
+    F4* shm;     // (h, m, kWCoef / h^3, kWDCoef / h^4)
+    F4* s2;      // (rho, p / rho^2, h, kWDCoef / h^4)
     uint32_t* sscene;
     uint32_t* ncount;
-    uint32_t* list;  // [(listCap + 1) * pCap], entry-major; row listCap is spare
-    uint32_t listCap;
+    // particles whose neighbours overflow the list (sorted slots), and their
+    // number (reset by kPrep each step)
+    uint32_t* overflowList;
+    uint32_t* overflowCount;
+    uint32_t* list;  // [listCap * pCap] in 4-entry chunks: ((k / 4) * pCap + t) * 4 + k % 4
+    uint32_t listCap;  // a multiple of 4
     // per aligned group of kGroup sorted slots: bounding box and max (2h)^2
     F4* gbox0;  // (min x, min y, min z, max (2h)^2)
     F4* gbox1;  // (max x, max y, max z, 0)
@@ -83,6 +87,10 @@ struct Ctx
     F4* bs0;  // (x, y, z, psi)
     float* bvisc;
     uint32_t* bcellOff;  // [cCap]: absolute boundary slot of each cell start
+    // [cCap]: bounding box of the boundary particles keyed to each cell
+    // (min xyz, max xyz; empty cells hold an inverted box)
+    F4* bbox0;
+    F4* bbox1;
 
     // refinement scratch (dense-indexed)
     uint64_t* ckeysA;
@@ -117,6 +125,7 @@ void launchPrep(const Ctx& c, hipStream_t s);
 void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchSceneRanges(const Ctx& c, hipStream_t s);
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 

@@ -148,6 +148,7 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
     if (threadIdx.x == 0)
     {
         *c.nTotal = carry;
+        *c.overflowCount = 0U;
     }
 }
 
@@ -261,7 +262,11 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
             const float r2 = r * r;
             c.s0[t] = F4 {p.x, p.y, p.z, r2};
             c.s1[t] = F4 {v.x, v.y, v.z, mm};
-            c.shm[t] = float2 {hh, mm};
+            // the per-particle factors of wendlandKernel / wendlandDerivativeKernel
+            // (WendlandKernel.cuh:39-52, 69-84), same expressions and rounding
+            const float h3 = (hh * hh) * hh;
+            const float h4 = h3 * hh;
+            c.shm[t] = F4 {hh, mm, kWCoef / h3, kWDCoef / h4};
             c.sscene[t] = c.cellOrder[k];
             lx = hx = p.x;
             ly = hy = p.y;
@@ -295,6 +300,19 @@ __device__ __forceinline__ bool groupMayHit(const F4& b0, const F4& b1, const F4
     const float ez = fmaxf(fmaxf(b0.z - q.z, q.z - b1.z), 0.0F);
     const float d2 = (ex * ex + ey * ey) + ez * ez;
     return d2 <= b0.w * 1.0001F;
+}
+
+// False only if every point of the box [b0, b1] is provably farther from q
+// than sqrt(r2): the box distance is a lower bound of every member's
+// distance, and the margin covers its FP32 rounding. An empty (inverted) box
+// is never reached.
+__device__ __forceinline__ bool boxMayReach(const F4& b0, const F4& b1, const F4& q, float r2)
+{
+    const float ex = fmaxf(fmaxf(b0.x - q.x, q.x - b1.x), 0.0F);
+    const float ey = fmaxf(fmaxf(b0.y - q.y, q.y - b1.y), 0.0F);
+    const float ez = fmaxf(fmaxf(b0.z - q.z, q.z - b1.z), 0.0F);
+    const float d2 = (ex * ex + ey * ey) + ez * ez;
+    return d2 <= r2 * 1.0001F;
 }
 
 // Calls f(j, o) for every slot j in [lo, hi) (ascending) with o = s0[j] and
@@ -387,23 +405,6 @@ __device__ __forceinline__ int waveMaxI(int v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-__device__ __forceinline__ float waveMinF(float v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        v = fminf(v, __shfl_xor(v, off, 64));
-    }
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
-
-__device__ __forceinline__ float waveMaxF(float v)
-{
-    for (int off = 32; off > 0; off >>= 1)
-    {
-        v = fmaxf(v, __shfl_xor(v, off, 64));
-    }
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-}
 
 // Wave-uniform walk of the fluid grid for the 64 queries of a wave (all of one
 // scene). Every lane visits the union of the wave's candidate cells in the
@@ -411,7 +412,8 @@ __device__ __forceinline__ float waveMaxF(float v)
 // a candidate only if its cell lies in the lane's own candidate box and
 // d^2 <= (2 h_j)^2, so each lane sees exactly its reference neighbour
 // sequence. Candidates are read with wave-uniform addresses, and groups of
-// kGroup slots too far from every query of the wave are skipped.
+// kGroup slots out of reach of every lane whose box holds the cell are
+// skipped.
 // Read-only views in the constant address space: loads with wave-uniform
 // addresses then go through the scalar cache (the arrays are not written by
 // the kernels that read them this way).
@@ -434,12 +436,6 @@ __device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint3
     const int ux1 = waveMaxI(valid ? b.x1 : -1);
     const int uy1 = waveMaxI(valid ? b.y1 : -1);
     const int uz1 = waveMaxI(valid ? b.z1 : -1);
-    const float wx0 = waveMinF(valid ? q.x : FLT_MAX);
-    const float wy0 = waveMinF(valid ? q.y : FLT_MAX);
-    const float wz0 = waveMinF(valid ? q.z : FLT_MAX);
-    const float wx1 = waveMaxF(valid ? q.x : -FLT_MAX);
-    const float wy1 = waveMaxF(valid ? q.y : -FLT_MAX);
-    const float wz1 = waveMaxF(valid ? q.z : -FLT_MAX);
     const int gx = sc.gx;
     const int gxgy = sc.gx * sc.gy;
     const uint32_t base = sc.cellBase;
@@ -467,11 +463,15 @@ __device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint3
                 {
                     const F4 b0 = ldc(gb0, g);
                     const F4 b1 = ldc(gb1, g);
-                    const float ex = fmaxf(fmaxf(b0.x - wx1, wx0 - b1.x), 0.0F);
-                    const float ey = fmaxf(fmaxf(b0.y - wy1, wy0 - b1.y), 0.0F);
-                    const float ez = fmaxf(fmaxf(b0.z - wz1, wz0 - b1.z), 0.0F);
+                    // skip the group unless some lane whose box holds this cell
+                    // may reach a member (distance from the lane's query to the
+                    // group box against the group's largest support, with a
+                    // margin far above FP32 rounding)
+                    const float ex = fmaxf(fmaxf(b0.x - q.x, q.x - b1.x), 0.0F);
+                    const float ey = fmaxf(fmaxf(b0.y - q.y, q.y - b1.y), 0.0F);
+                    const float ez = fmaxf(fmaxf(b0.z - q.z, q.z - b1.z), 0.0F);
                     const float d2g = (ex * ex + ey * ey) + ez * ez;
-                    if (!(d2g <= b0.w * 1.0001F))
+                    if (__ballot((inbox && d2g <= b0.w * 1.0001F) ? 1 : 0) == 0ULL)
                     {
                         continue;
                     }
@@ -521,18 +521,68 @@ __device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint3
     }
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
+// XCD i mod 8), and each XCD has its own L2. The neighbour-list consumers
+// renumber their workgroups so that each XCD takes one contiguous eighth of
+// the sorted particles: neighbouring particles (which gather the same
+// neighbour data) then share an L2. Results do not depend on the numbering.
+// (kNeighbors reads its candidates through the scalar cache and measured
+// faster without the renumbering.)
+__device__ __forceinline__ uint32_t xcdBlock()
+{
+    constexpr uint32_t kXcds = 8;
+    const uint32_t per = gridDim.x / kXcds;
+    const uint32_t b = blockIdx.x;
+    if (b >= per * kXcds)
+    {
+        return b;
+    }
+    return (b % kXcds) * per + b / kXcds;
+}
+
+// Neighbour-list entries whose data are loaded together before the pair math
+// (memory-level parallelism against register footprint).
+#ifndef ASPH_DENSITY_BATCH
+#define ASPH_DENSITY_BATCH 8
+#endif
+#ifndef ASPH_FORCE_BATCH
+#define ASPH_FORCE_BATCH 4
+#endif
+constexpr uint32_t kDensityBatch = ASPH_DENSITY_BATCH;  // a multiple of 4
+constexpr uint32_t kForceBatch = ASPH_FORCE_BATCH;      // a multiple of 4
+
 // -------------------------------------------------------------- density --
 // computeDensities (Algorithm.cu:17-54) and computeBoundaryDensityContribution
 // (BoundaryAlgorithm.cu:15-51). Phase 1 finds each particle's accepted
-// neighbours in the reference's visiting order and stores their slots in an
-// entry-major neighbour list; phase 2 accumulates the density over that list
-// in the same order with every lane active.
-__global__ __launch_bounds__(256) void kNeighbors(Ctx c)
+// neighbours in the reference's visiting order and stores their slots in the
+// neighbour list; phase 2 accumulates the density over that list in the same
+// order with every lane active.
+//
+// List layout: chunks of kChunk = 4 entries, chunk-major then particle:
+// entry k of sorted slot t is list[((k / 4) * pCap + t) * 4 + k % 4]. A
+// particle's chunk is one 16-byte segment, written whole from an LDS staging
+// ring; at a given k the wave's lanes read 64 adjacent segments (1 KiB,
+// coalesced), four entries per 16-byte load.
+constexpr uint32_t kChunk = 4;
+
+__device__ __forceinline__ size_t chunkStride(const Ctx& c) { return static_cast<size_t>(c.pCap) * kChunk; }
+
+__device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint32_t k)
 {
+    return c.list + (static_cast<size_t>(k / kChunk) * c.pCap + t) * kChunk + (k % kChunk);
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
+{
+    // staging ring: kChunk entries per thread, entry-major ([e][thread]) so
+    // that the lanes' appends fall in distinct banks
+    extern __shared__ uint32_t stage[];
     const uint32_t n = *c.nTotal;
     const uint32_t nPad = (n + 63U) & ~63U;
     const uint32_t K = c.listCap;
-    const size_t LS = c.pCap;
+    const size_t CS = chunkStride(c);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t bd = blockDim.x;
     for (uint32_t tw = blockIdx.x * blockDim.x + threadIdx.x; tw < nPad; tw += gridDim.x * blockDim.x)
     {
         const bool valid = tw < n;
@@ -547,131 +597,199 @@ __global__ __launch_bounds__(256) void kNeighbors(Ctx c)
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
         uint32_t cnt = 0;
-        uint32_t* lst = c.list + t;
+        uint32_t* lbase = c.list + static_cast<size_t>(t) * kChunk;
+        // writes the staged chunk `ch` of this thread as one 16-byte segment
+        auto flush = [&](uint32_t ch) {
+            *reinterpret_cast<uint4*>(lbase + ch * CS) =
+                uint4 {stage[tid], stage[bd + tid], stage[2U * bd + tid], stage[3U * bd + tid]};
+        };
+        auto append = [&](uint32_t j) {
+            if (cnt < K)
+            {
+                const uint32_t slot = cnt % kChunk;
+                stage[slot * bd + tid] = j;
+                if (slot == kChunk - 1U)
+                {
+                    flush(cnt / kChunk);
+                }
+            }
+            ++cnt;
+        };
         if (oneScene)
         {
-            waveWalk(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, [&](uint32_t j) {
-                lst[min(cnt, K) * LS] = j;  // row K is spare: entries past the capacity land there
-                ++cnt;
-            });
+            waveWalk(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, append);
         }
         else if (valid)
         {
-            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
-                if (cnt < K)
-                {
-                    lst[cnt * LS] = j;
-                }
-                ++cnt;
-            });
+            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) { append(j); });
         }
         if (valid)
         {
+            if (cnt <= K && (cnt % kChunk) != 0U)
+            {
+                flush(cnt / kChunk);  // partial last chunk (entries past cnt are never read)
+            }
             c.ncount[t] = cnt;
+            if (cnt > K)
+            {
+                c.overflowList[atomicAdd(c.overflowCount, 1U)] = t;
+            }
         }
     }
 }
 
+// Density of a particle whose neighbours overflow its list: the walk again,
+// accumulating in the same order. Kept out of line (rare) so that the list
+// loop of kDensity keeps a small register footprint.
+__device__ __forceinline__ float densityWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
+                                          const F4& q)
+{
+    float rho = 0.0F;
+    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float d2) {
+        const F4 hm = c.shm[j];
+        rho = rho + hm.y * wendlandC(sqrtf(d2), hm.x, hm.z);
+    });
+    return rho;
+}
+
+// computeBoundaryDensityContribution (BoundaryAlgorithm.cu:15-51) for one
+// particle: sum of psi_b W(d, h_i) over boundary particles within 2 h_i.
+__device__ __forceinline__ float boundaryDensity(const Ctx& c, const SceneDev& sc, const CellBox& b, const F4& q,
+                                              float hi)
+{
+    const float R = 2.0F * hi;
+    const float r2 = R * R;
+    const int gxgy = sc.gx * sc.gy;
+    float rhoB = 0.0F;
+    for (int x = b.x0; x <= b.x1; ++x)
+    {
+        for (int y = b.y0; y <= b.y1; ++y)
+        {
+            for (int z = b.z0; z <= b.z1; ++z)
+            {
+                const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
+                const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
+                const uint32_t hiS = c.bcellOff[cell + 1U];
+                if (lo >= hiS || !boxMayReach(c.bbox0[cell], c.bbox1[cell], q, r2))
+                {
+                    continue;
+                }
+                for (uint32_t j = lo; j < hiS; ++j)
+                {
+                    const F4 o = c.bs0[j];
+                    const float dx = o.x - q.x;
+                    const float dy = o.y - q.y;
+                    const float dz = o.z - q.z;
+                    const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                    if (d2 <= r2)
+                    {
+                        rhoB = rhoB + o.w * wendland(sqrtf(d2), hi);
+                    }
+                }
+            }
+        }
+    }
+    return rhoB;
+}
+
+// Adds the boundary contribution (launched unconditionally by the
+// reference), then Tait pressure; stores (rho, p / rho^2, h, kWDCoef / h^4).
+__device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, const CellBox& b, const F4& q,
+                                              float hi, uint32_t t, float rho)
+{
+    float rhoB = 0.0F;
+    if (sc.bCount > 0U && touchesEdge(sc, b))
+    {
+        rhoB = boundaryDensity(c, sc, b, q, hi);
+    }
+    rho = rho + rhoB;
+    const float p = tait(rho, sc);
+    c.s2[t] = F4 {rho, p / (rho * rho), hi, c.shm[t].w};
+}
+
 // Phase 2 of computeDensities over the neighbour list, plus
-// computeBoundaryDensityContribution; also precomputes p and p / rho^2.
+// computeBoundaryDensityContribution; also precomputes p / rho^2. Particles
+// whose neighbours overflow the list are left to kDensityOverflow.
 __global__ __launch_bounds__(256) void kDensity(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t K = c.listCap;
-    const size_t LS = c.pCap;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    for (uint32_t t = xcdBlock() * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
         const uint32_t s = c.sscene[t];
         const SceneDev& sc = c.scenes[s];
-        const uint32_t lost = c.states[s].lostEnd;
+        const uint32_t cnt = c.ncount[t];
+        if (cnt > K)
+        {
+            continue;  // kDensityOverflow
+        }
         const F4 q = c.s0[t];
         const float hi = c.shm[t].x;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        const uint32_t cnt = c.ncount[t];
-        const uint32_t* lst = c.list + t;
         float rho = 0.0F;
-        if (cnt <= K)
         {
             uint32_t e = 0;
-            for (; e + 4U <= cnt; e += 4U)
+            for (; e + kDensityBatch <= cnt; e += kDensityBatch)
             {
-                uint32_t j[4];
-                F4 o[4];
-                float2 hm[4];
+                uint32_t j[kDensityBatch];
+                F4 o[kDensityBatch];
+                F4 hm[kDensityBatch];
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (uint32_t u = 0; u < kDensityBatch; u += 4U)
                 {
-                    j[u] = lst[(e + u) * LS];
+                    const uint4 v = *reinterpret_cast<const uint4*>(listAt(c, t, e + u));
+                    j[u] = v.x;
+                    j[u + 1U] = v.y;
+                    j[u + 2U] = v.z;
+                    j[u + 3U] = v.w;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (uint32_t u = 0; u < kDensityBatch; ++u)
                 {
                     o[u] = c.s0[j[u]];
                     hm[u] = c.shm[j[u]];
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
+                for (uint32_t u = 0; u < kDensityBatch; ++u)
                 {
                     const float dx = o[u].x - q.x;
                     const float dy = o[u].y - q.y;
                     const float dz = o[u].z - q.z;
                     const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
-                    rho = rho + hm[u].y * wendland(d, hm[u].x);
+                    rho = rho + hm[u].y * wendlandC(d, hm[u].x, hm[u].z);
                 }
             }
             for (; e < cnt; ++e)
             {
-                const uint32_t jj = lst[e * LS];
+                const uint32_t jj = *listAt(c, t, e);
                 const F4 o = c.s0[jj];
-                const float2 hm = c.shm[jj];
+                const F4 hm = c.shm[jj];
                 const float dx = o.x - q.x;
                 const float dy = o.y - q.y;
                 const float dz = o.z - q.z;
-                rho = rho + hm.y * wendland(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), hm.x);
+                rho = rho + hm.y * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), hm.x, hm.z);
             }
         }
-        else
-        {
-            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float d2) {
-                const float2 hm = c.shm[j];
-                rho = rho + hm.y * wendland(sqrtf(d2), hm.x);
-            });
-        }
-        // boundary contribution (always added, as the reference launches it unconditionally)
-        float rhoB = 0.0F;
-        if (sc.bCount > 0U && touchesEdge(sc, b))
-        {
-            const float r2 = R * R;
-            const int gxgy = sc.gx * sc.gy;
-            for (int x = b.x0; x <= b.x1; ++x)
-            {
-                for (int y = b.y0; y <= b.y1; ++y)
-                {
-                    for (int z = b.z0; z <= b.z1; ++z)
-                    {
-                        const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                        const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
-                        const uint32_t hiS = c.bcellOff[cell + 1U];
-                        for (uint32_t j = lo; j < hiS; ++j)
-                        {
-                            const F4 o = c.bs0[j];
-                            const float dx = o.x - q.x;
-                            const float dy = o.y - q.y;
-                            const float dz = o.z - q.z;
-                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            if (d2 <= r2)
-                            {
-                                rhoB = rhoB + o.w * wendland(sqrtf(d2), hi);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        rho = rho + rhoB;
-        const float p = tait(rho, sc);
-        c.s2[t] = F4 {rho, p, p / (rho * rho), hi};
+        finishDensity(c, sc, b, q, hi, t, rho);
+    }
+}
+
+// Density of the particles whose neighbours overflow their list: the walk
+// again, accumulating in the same order.
+__global__ __launch_bounds__(256) void kDensityOverflow(Ctx c)
+{
+    const uint32_t m = *c.overflowCount;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    {
+        const uint32_t t = c.overflowList[i];
+        const uint32_t s = c.sscene[t];
+        const SceneDev& sc = c.scenes[s];
+        const F4 q = c.s0[t];
+        const float hi = c.shm[t].x;
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
+        const float rho = densityWalk(c, sc, c.states[s].lostEnd, b, q);
+        finishDensity(c, sc, b, q, hi, t, rho);
     }
 }
 
@@ -686,7 +804,7 @@ struct PairAcc
 // One fluid pair of computePressureAccelerations (Algorithm.cu:72-105),
 // computeViscosityAccelerations (:129-171) and
 // computeSurfaceTensionAccelerations (:192-218), accumulated separately.
-// o = s0[j], nj = s1[j] (velocity, mass), rj = s2[j] (rho, p, p/rho^2, h).
+// o = s0[j], nj = s1[j] (velocity, mass), rj = s2[j] (rho, p/rho^2, h, kWDCoef/h^4).
 __device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& rj, const F4& q, const F4& vi,
                                           float rhoi, float prri, float nuNum, float epsH2, bool doVisc, bool doST,
                                           PairAcc& A)
@@ -706,8 +824,8 @@ __device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& r
         dz = oz / d;
     }
     const float mj = nj.w;
-    const float pterm = prri + rj.z;
-    const float dw = wendlandGrad(d, rj.w);
+    const float pterm = prri + rj.y;
+    const float dw = wendlandGradC(d, rj.z, rj.w);
     A.apx = A.apx - ((mj * dx) * dw) * pterm;
     A.apy = A.apy - ((mj * dy) * dw) * pterm;
     A.apz = A.apz - ((mj * dz) * dw) * pterm;
@@ -729,185 +847,259 @@ __device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& r
     }
     if (doST)
     {
-        const float sw = mj * wendland(d, rj.w);
+        const float sw = mj * wendland(d, rj.z);
         A.asx = A.asx + sw * dx;
         A.asy = A.asy + sw * dy;
         A.asz = A.asz + sw * dz;
     }
 }
 
+// Fluid pair forces of a particle whose neighbours overflow its list: the
+// walk again, in the same order. Out of line (rare), like densityWalk.
+__device__ __forceinline__ PairAcc forceWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
+                                          const F4& q, const F4& vi, float rhoi, float prri, float nuNum, float epsH2,
+                                          bool doVisc, bool doST, uint32_t t)
+{
+    PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
+    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4& o, float) {
+        if (j != t)
+        {
+            forcePair(o, c.s1[j], c.s2[j], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+        }
+    });
+    return A;
+}
+
+struct BoundaryAcc
+{
+    float bpx, bpy, bpz;
+    float bfx, bfy, bfz;
+};
+
+// computeBoundaryPressureAcceleration and computeBoundaryFrictionAcceleration
+// (BoundaryAlgorithm.cu:53-162) for one particle; ri = s2[t].
+__device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDev& sc, const CellBox& b, const F4& q,
+                                                  const F4& vi, const F4& ri, float epsH2)
+{
+    const float hi = ri.z;
+    const float rhoi = ri.x;
+    const float prri = ri.y;
+    const float R = 2.0F * hi;
+    const float r2 = R * R;
+    const int gxgy = sc.gx * sc.gy;
+    BoundaryAcc B {0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
+    for (int x = b.x0; x <= b.x1; ++x)
+    {
+        for (int y = b.y0; y <= b.y1; ++y)
+        {
+            for (int z = b.z0; z <= b.z1; ++z)
+            {
+                const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
+                const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
+                const uint32_t hiS = c.bcellOff[cell + 1U];
+                if (lo >= hiS || !boxMayReach(c.bbox0[cell], c.bbox1[cell], q, r2))
+                {
+                    continue;
+                }
+                for (uint32_t j = lo; j < hiS; ++j)
+                {
+                    const F4 o = c.bs0[j];
+                    const float ox = q.x - o.x;
+                    const float oy = q.y - o.y;
+                    const float oz = q.z - o.z;
+                    const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
+                    if (d2 > r2)
+                    {
+                        continue;
+                    }
+                    const float d = sqrtf(d2);
+                    float dx = 0.0F;
+                    float dy = 1.0F;
+                    float dz = 0.0F;
+                    if (d > 0.0F)
+                    {
+                        dx = ox / d;
+                        dy = oy / d;
+                        dz = oz / d;
+                    }
+                    const float psi = o.w;
+                    const float dw = wendlandGradC(d, hi, ri.w);
+                    const float sp = (psi * prri) * dw;
+                    B.bpx = B.bpx - sp * dx;
+                    B.bpy = B.bpy - sp * dy;
+                    B.bpz = B.bpz - sp * dz;
+                    const float comp = dot3w(vi.x, vi.y, vi.z, ox, oy, oz);
+                    if (comp < 0.0F)
+                    {
+                        const float f = c.bvisc[j];
+                        const float nu = ((f * hi) * sc.c0) / (2.0F * rhoi);
+                        const float piv = ((-nu) * comp) / (d2 + epsH2);
+                        B.bfx = B.bfx - ((dx * psi) * piv) * dw;
+                        B.bfy = B.bfy - ((dy * psi) * piv) * dw;
+                        B.bfz = B.bfz - ((dz * psi) * piv) * dw;
+                    }
+                }
+            }
+        }
+    }
+    return B;
+}
+
 // computeExternalAccelerations + fluid pair forces + boundary pressure and
 // friction (BoundaryAlgorithm.cu:53-162), added to g in the reference's launch
 // order (SphSimulation.cu:110-115), then the second half kick (:117) unless the
 // scene refines this step (the kick then follows the refinement).
+// Fluid pair sums of particle t are in A; adds gravity, the fluid terms,
+// the boundary terms, stores acc and rho, then the second half kick.
+__device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, const SceneState& st, const CellBox& b,
+                                            const F4& q, const F4& vi, const F4& ri, float epsH2, bool doVisc,
+                                            bool doST, uint32_t t, const PairAcc& A)
+{
+    float ax = st.gEff[0];
+    float ay = st.gEff[1];
+    float az = st.gEff[2];
+    ax = ax + A.apx;
+    ay = ay + A.apy;
+    az = az + A.apz;
+    if (doVisc)
+    {
+        ax = ax + A.avx;
+        ay = ay + A.avy;
+        az = az + A.avz;
+    }
+    if (doST)
+    {
+        const float k = sc.sigma / vi.w;
+        ax = ax + k * A.asx;
+        ay = ay + k * A.asy;
+        az = az + k * A.asz;
+    }
+    BoundaryAcc B {0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
+    if (sc.bCount > 0U && touchesEdge(sc, b))
+    {
+        B = boundaryForce(c, sc, b, q, vi, ri, epsH2);
+    }
+    ax = ax + B.bpx;
+    ay = ay + B.bpy;
+    az = az + B.bpz;
+    ax = ax + B.bfx;
+    ay = ay + B.bfy;
+    az = az + B.bfz;
+    const uint32_t i = c.sortedVals[t];
+    c.acc[i] = F4 {ax, ay, az, 0.0F};
+    c.rho[i] = ri.x;
+    if (st.refineNow == 0U)
+    {
+        const float hdt = sc.halfDt;
+        float vx = vi.x + ax * hdt;
+        float vy = vi.y + ay * hdt;
+        float vz = vi.z + az * hdt;
+        float vw = 0.0F;
+        const float len = sqrtf(dot4(vx, vy, vz, vw));
+        if (len > sc.vmax)
+        {
+            const float k = sc.vmax / len;
+            vx = vx * k;
+            vy = vy * k;
+            vz = vz * k;
+            vw = vw * k;
+        }
+        c.vel[i] = F4 {vx, vy, vz, vw};
+    }
+}
+
 __global__ __launch_bounds__(256) void kForce(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t K = c.listCap;
-    const size_t LS = c.pCap;
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    for (uint32_t t = xcdBlock() * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
+        const uint32_t cnt = c.ncount[t];
+        if (cnt > K)
+        {
+            continue;  // kForceOverflow
+        }
         const uint32_t s = c.sscene[t];
         const SceneDev& sc = c.scenes[s];
         const SceneState& st = c.states[s];
         const F4 q = c.s0[t];
         const F4 vi = c.s1[t];
         const F4 ri = c.s2[t];
-        const float hi = ri.w;
+        const float hi = ri.z;
         const float rhoi = ri.x;
-        const float prri = ri.z;
-        const uint32_t cnt = c.ncount[t];
+        const float prri = ri.y;
         const bool doVisc = sc.alpha != 0.0F;
         const bool doST = sc.sigma != 0.0F;
         const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
         const float epsH2 = (0.01F * hi) * hi;
-        const float R = 2.0F * hi;
-        const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
         PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
-        if (cnt <= K)
+        uint32_t e = 0;
+        for (; e + kForceBatch <= cnt; e += kForceBatch)
         {
-            const uint32_t* lst = c.list + t;
-            uint32_t e = 0;
-            for (; e + 2U <= cnt; e += 2U)
+            uint32_t j[kForceBatch];
+            F4 o[kForceBatch];
+            F4 nj[kForceBatch];
+            F4 rj[kForceBatch];
+#pragma unroll
+            for (uint32_t u = 0; u < kForceBatch; u += 4U)
             {
-                const uint32_t j0 = lst[e * LS];
-                const uint32_t j1 = lst[(e + 1U) * LS];
-                const F4 o0 = c.s0[j0];
-                const F4 n0 = c.s1[j0];
-                const F4 r0 = c.s2[j0];
-                const F4 o1 = c.s0[j1];
-                const F4 n1 = c.s1[j1];
-                const F4 r1 = c.s2[j1];
-                if (j0 != t)
-                {
-                    forcePair(o0, n0, r0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                }
-                if (j1 != t)
-                {
-                    forcePair(o1, n1, r1, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                }
+                const uint4 v = *reinterpret_cast<const uint4*>(listAt(c, t, e + u));
+                j[u] = v.x;
+                j[u + 1U] = v.y;
+                j[u + 2U] = v.z;
+                j[u + 3U] = v.w;
             }
-            if (e < cnt)
+#pragma unroll
+            for (uint32_t u = 0; u < kForceBatch; ++u)
             {
-                const uint32_t j0 = lst[e * LS];
-                if (j0 != t)
-                {
-                    forcePair(c.s0[j0], c.s1[j0], c.s2[j0], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                }
+                o[u] = c.s0[j[u]];
+                nj[u] = c.s1[j[u]];
+                rj[u] = c.s2[j[u]];
             }
-        }
-        else
-        {
-            walkFluid(c, sc, st.lostEnd, b, q, [&](uint32_t j, const F4& o, float) {
-                if (j != t)
-                {
-                    forcePair(o, c.s1[j], c.s2[j], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                }
-            });
-        }
-        float ax = st.gEff[0];
-        float ay = st.gEff[1];
-        float az = st.gEff[2];
-        ax = ax + A.apx;
-        ay = ay + A.apy;
-        az = az + A.apz;
-        if (doVisc)
-        {
-            ax = ax + A.avx;
-            ay = ay + A.avy;
-            az = az + A.avz;
-        }
-        if (doST)
-        {
-            const float k = sc.sigma / vi.w;
-            ax = ax + k * A.asx;
-            ay = ay + k * A.asy;
-            az = az + k * A.asz;
-        }
-        float bpx = 0.F, bpy = 0.F, bpz = 0.F;
-        float bfx = 0.F, bfy = 0.F, bfz = 0.F;
-        if (sc.bCount > 0U && touchesEdge(sc, b))
-        {
-            const float r2 = R * R;
-            const int gxgy = sc.gx * sc.gy;
-            for (int x = b.x0; x <= b.x1; ++x)
+#pragma unroll
+            for (uint32_t u = 0; u < kForceBatch; ++u)
             {
-                for (int y = b.y0; y <= b.y1; ++y)
+                if (j[u] != t)
                 {
-                    for (int z = b.z0; z <= b.z1; ++z)
-                    {
-                        const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                        const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
-                        const uint32_t hiS = c.bcellOff[cell + 1U];
-                        for (uint32_t j = lo; j < hiS; ++j)
-                        {
-                            const F4 o = c.bs0[j];
-                            const float ox = q.x - o.x;
-                            const float oy = q.y - o.y;
-                            const float oz = q.z - o.z;
-                            const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
-                            if (d2 > r2)
-                            {
-                                continue;
-                            }
-                            const float d = sqrtf(d2);
-                            float dx = 0.0F;
-                            float dy = 1.0F;
-                            float dz = 0.0F;
-                            if (d > 0.0F)
-                            {
-                                dx = ox / d;
-                                dy = oy / d;
-                                dz = oz / d;
-                            }
-                            const float psi = o.w;
-                            const float dw = wendlandGrad(d, hi);
-                            const float sp = (psi * prri) * dw;
-                            bpx = bpx - sp * dx;
-                            bpy = bpy - sp * dy;
-                            bpz = bpz - sp * dz;
-                            const float comp = dot3w(vi.x, vi.y, vi.z, ox, oy, oz);
-                            if (comp < 0.0F)
-                            {
-                                const float f = c.bvisc[j];
-                                const float nu = ((f * hi) * sc.c0) / (2.0F * rhoi);
-                                const float piv = ((-nu) * comp) / (d2 + epsH2);
-                                bfx = bfx - ((dx * psi) * piv) * dw;
-                                bfy = bfy - ((dy * psi) * piv) * dw;
-                                bfz = bfz - ((dz * psi) * piv) * dw;
-                            }
-                        }
-                    }
+                    forcePair(o[u], nj[u], rj[u], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
                 }
             }
         }
-        ax = ax + bpx;
-        ay = ay + bpy;
-        az = az + bpz;
-        ax = ax + bfx;
-        ay = ay + bfy;
-        az = az + bfz;
-        const uint32_t i = c.sortedVals[t];
-        c.acc[i] = F4 {ax, ay, az, 0.0F};
-        c.rho[i] = rhoi;
-        if (st.refineNow == 0U)
+        for (; e < cnt; ++e)
         {
-            const float hdt = sc.halfDt;
-            float vx = vi.x + ax * hdt;
-            float vy = vi.y + ay * hdt;
-            float vz = vi.z + az * hdt;
-            float vw = 0.0F;
-            const float len = sqrtf(dot4(vx, vy, vz, vw));
-            if (len > sc.vmax)
+            const uint32_t j0 = *listAt(c, t, e);
+            if (j0 != t)
             {
-                const float k = sc.vmax / len;
-                vx = vx * k;
-                vy = vy * k;
-                vz = vz * k;
-                vw = vw * k;
+                forcePair(c.s0[j0], c.s1[j0], c.s2[j0], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
             }
-            c.vel[i] = F4 {vx, vy, vz, vw};
         }
+        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, t, A);
+    }
+}
+
+// kForce for the particles whose neighbours overflow their list: the walk
+// again, in the same order.
+__global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
+{
+    const uint32_t m = *c.overflowCount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+    {
+        const uint32_t t = c.overflowList[k];
+        const uint32_t s = c.sscene[t];
+        const SceneDev& sc = c.scenes[s];
+        const SceneState& st = c.states[s];
+        const F4 q = c.s0[t];
+        const F4 vi = c.s1[t];
+        const F4 ri = c.s2[t];
+        const float hi = ri.z;
+        const bool doVisc = sc.alpha != 0.0F;
+        const bool doST = sc.sigma != 0.0F;
+        const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
+        const float epsH2 = (0.01F * hi) * hi;
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
+        const PairAcc A = forceWalk(c, sc, st.lostEnd, b, q, vi, ri.x, ri.y, nuNum, epsH2, doVisc, doST, t);
+        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, t, A);
     }
 }
 
@@ -1114,6 +1306,28 @@ __device__ float mergeCriterion(const Ctx& c, const SceneDev& sc, const SceneSta
     return -1.0F;
 }
 
+// *ctr += pred for the active lanes, where ctr is a per-scene counter of
+// scene s: one atomic per wave when every active lane belongs to the same
+// scene (the usual case), one per lane otherwise. Counting only, so the
+// result does not depend on the order of the additions.
+__device__ __forceinline__ void countAdd(uint32_t* ctr, uint32_t s, bool pred)
+{
+    const uint32_t s0 = __builtin_amdgcn_readfirstlane(s);
+    if (__ballot((s != s0) ? 1 : 0) == 0ULL)
+    {
+        const uint64_t m = __ballot(pred ? 1 : 0);
+        const uint64_t active = __ballot(1);
+        if (m != 0ULL && __lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(active)) - 1))
+        {
+            atomicAdd(ctr, static_cast<uint32_t>(__popcll(m)));
+        }
+    }
+    else if (pred)
+    {
+        atomicAdd(ctr, 1U);
+    }
+}
+
 __device__ __forceinline__ bool mergeParticipant(const SceneDev& sc, const SceneState& st)
 {
     return (sc.flags & kSceneActive) != 0U && st.refineNow != 0U && st.countBefore > 1U;
@@ -1142,10 +1356,7 @@ __global__ __launch_bounds__(256) void kMergeCriteria(Ctx c)
         const float v = mergeCriterion(c, sc, st, sc.pOff + l);
         c.ckeysA[d] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(orderedBits(v));
         c.cvalsA[d] = l;
-        if (v > 0.0F)
-        {
-            atomicAdd(&st.positives, 1U);
-        }
+        countAdd(&st.positives, s, v > 0.0F);
     }
 }
 
@@ -1249,12 +1460,13 @@ __global__ __launch_bounds__(256) void kResolvePairs(Ctx c)
         {
             continue;
         }
-        if (c.cand[st.denseOff + cc] == l && l < cc)
+        const bool pair = c.cand[st.denseOff + cc] == l && l < cc;
+        if (pair)
         {
             c.keep[d] = 1U;
             c.removal[st.denseOff + cc] = 1U;
-            atomicAdd(&st.mergePairs, 1U);
         }
+        countAdd(&st.mergePairs, s, pair);
     }
 }
 
@@ -1417,10 +1629,7 @@ __global__ __launch_bounds__(256) void kSplitCriteria(Ctx c)
         const float v = splitCriterion(c, sc, st, sc.pOff + l);
         c.ckeysA[d] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(~orderedBits(v));
         c.cvalsA[d] = l;
-        if (v > 0.0F)
-        {
-            atomicAdd(&st.positives, 1U);
-        }
+        countAdd(&st.positives, s, v > 0.0F);
     }
 }
 
@@ -1680,17 +1889,34 @@ void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     hipLaunchKernelGGL(kGather, dim3(gridFor(c.nMax, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
 }
 
+// Overflow kernels: a grid that covers a queue of any length with enough
+// parallelism (refined regions can overflow a large share of the particles)
+// while costing little when the queue is empty.
+__host__ __forceinline__ uint32_t overflowGrid(const Ctx& c)
+{
+    constexpr uint32_t kMaxBlocks = 2048;
+    const uint32_t g = gridFor(c.nMax, 256);
+    return g < kMaxBlocks ? g : kMaxBlocks;
+}
+
+void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
+{
+    const uint32_t b = lc.blockNeighbor;
+    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, b)), dim3(b), b * kChunk * sizeof(uint32_t), s, c);
+}
+
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
     const uint32_t b = lc.blockNeighbor;
-    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
     hipLaunchKernelGGL(kDensity, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    hipLaunchKernelGGL(kDensityOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);
 }
 
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
     const uint32_t b = lc.blockNeighbor;
     hipLaunchKernelGGL(kForce, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    hipLaunchKernelGGL(kForceOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);
 }
 
 void launchMergeCriteria(const Ctx& c, const LaunchCfg& lc, hipStream_t s)

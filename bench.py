@@ -91,6 +91,8 @@ def stage_profile(batch, steps: int) -> dict:
 
 
 def roofline_of(stage_ms: dict, particles: int, workload: str) -> dict:
+    # the density stage is the neighbour-list build plus the density kernel
+    stage_ms = dict(stage_ms, density=stage_ms.get("neighbors", 0.0) + stage_ms.get("density", 0.0))
     dom = max(("density", "force"), key=lambda k: stage_ms.get(k, 0.0))
     dur = stage_ms[dom] * 1e-3
     achieved = ALGO_BYTES[dom] * particles / dur / 1e9
@@ -138,7 +140,7 @@ def run_single(args) -> dict:
 
     scenes, workload = make_workload(args.workload, args.merged_h)
     s = scenes[0]
-    batch = Batch.for_scenes(scenes, graph=not args.no_graph)
+    batch = Batch.for_scenes(scenes, graph=not args.no_graph, list_capacity=args.list_capacity)
     sid = batch.add(s)
     log(f"workload {workload}: {s.count} particles")
     batch.step(args.warmup)
@@ -188,14 +190,14 @@ def run_batch(args) -> dict:
     total0 = sum(s.count for s in scenes)
     log(f"workload {workload}: {len(scenes)} scenes, {total0} particles, "
         f"{sum(s.adaptive for s in scenes)} adaptive ({time.perf_counter() - t0:.1f} s to build)")
-    tuning = Batch.for_scenes(scenes, graph=True)
+    tuning = Batch.for_scenes(scenes, graph=True, list_capacity=args.list_capacity)
     for s in scenes:
         tuning.add(s)
     tune = autotune(tuning)
     tuning.close()
     log("autotune:", json.dumps(tune))
 
-    batch = Batch.for_scenes(scenes, graph=True)
+    batch = Batch.for_scenes(scenes, graph=True, list_capacity=args.list_capacity)
     for s in scenes:
         batch.add(s)
     batch.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
@@ -220,7 +222,7 @@ def run_batch(args) -> dict:
     seq_particles = 0
     seq_elapsed = 0.0
     for k, s in enumerate(scenes):
-        b = Batch.for_scenes([s], graph=True)
+        b = Batch.for_scenes([s], graph=True, list_capacity=args.list_capacity)
         b.add(s)
         b.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
         b.step(args.warmup)
@@ -258,7 +260,7 @@ def run_serve(args) -> dict:
     budget = int(args.budget_gb * 1e9)
     reqs = serve.poisson_requests(args.rate, args.duration, seed=5)
     log(f"serving: {len(reqs)} requests at {args.rate}/s over {args.duration} s, budget {args.budget_gb} GB")
-    srv = serve.Server(budget, max_scenes=256)
+    srv = serve.Server(budget, max_scenes=256, list_capacity=args.list_capacity)
     # one small request first, so the first measured request does not pay
     # for library and kernel loading
     warm = [serve.Request(rid=-1, seed=1, steps=3, arrival=0.0)]
@@ -326,9 +328,15 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=0.6, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget")
+    ap.add_argument("--list-capacity", type=int, default=0,
+                    help="neighbour-list entries per particle (0: 128 for plain workloads, 384 with refinement)")
     ap.add_argument("--merged-h", choices=("kernel", "reference"), default="kernel",
                     help="merged smoothing length of adaptive scenes: kernel-normalised or the reference's")
     args = ap.parse_args()
+    if args.list_capacity == 0:
+        # refined regions hold up to ~13x denser daughters: particles there have
+        # a few hundred neighbours
+        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy") else 384
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build steps scenes on one "
               "GPU and leaves multi-GPU sharding out", file=sys.stderr)
@@ -351,6 +359,8 @@ def main() -> int:
         out = run_serve(args)
     else:
         out = run_single(args)
+    if args.workload != "copy":
+        out["config"]["list_capacity"] = args.list_capacity
     print(json.dumps(out), flush=True)
     return 0
 
