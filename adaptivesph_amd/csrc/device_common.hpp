@@ -17,7 +17,10 @@ namespace asph
 
 constexpr uint32_t kNoCandidate = 0xFFFFFFFFU;  // refinement/Common.cuh:119-122
 constexpr uint32_t kWave = 64;
-constexpr uint32_t kGroup = 8;  // sorted slots per culling box
+#ifndef ASPH_GROUP
+#define ASPH_GROUP 16
+#endif
+constexpr uint32_t kGroup = ASPH_GROUP;  // sorted slots per culling box (a power of two, at most 64)
 
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;

@@ -330,28 +330,31 @@ __device__ __forceinline__ void scanRange(const Ctx& c, uint32_t lo, uint32_t hi
         {
             continue;
         }
-        const uint32_t base = g * kGroup;
-        F4 o[kGroup];
-#pragma unroll
-        for (uint32_t u = 0; u < kGroup; ++u)
+        // the group's members in batches of 4 loads (a small register footprint)
+        for (uint32_t base = g * kGroup; base < (g + 1U) * kGroup; base += 4U)
         {
-            o[u] = c.s0[min(base + u, hi - 1U)];
-        }
+            F4 o[4];
 #pragma unroll
-        for (uint32_t u = 0; u < kGroup; ++u)
-        {
-            const uint32_t j = base + u;
-            if (j < lo || j >= hi)
+            for (uint32_t u = 0; u < 4U; ++u)
             {
-                continue;
+                o[u] = c.s0[min(base + u, hi - 1U)];
             }
-            const float dx = o[u].x - q.x;
-            const float dy = o[u].y - q.y;
-            const float dz = o[u].z - q.z;
-            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-            if (d2 <= o[u].w)
+#pragma unroll
+            for (uint32_t u = 0; u < 4U; ++u)
             {
-                f(j, o[u], d2);
+                const uint32_t j = base + u;
+                if (j < lo || j >= hi)
+                {
+                    continue;
+                }
+                const float dx = o[u].x - q.x;
+                const float dy = o[u].y - q.y;
+                const float dz = o[u].z - q.z;
+                const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                if (d2 <= o[u].w)
+                {
+                    f(j, o[u], d2);
+                }
             }
         }
     }
