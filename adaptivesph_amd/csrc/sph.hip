@@ -193,7 +193,23 @@ __global__ __launch_bounds__(1024) void kKickDriftKey(Ctx c)
         const uint32_t key = sc.cellBase + cellKey(sc, p.x, p.y, p.z);
         c.keysA[d] = key;
         c.valsA[d] = i;
-        atomicAdd(&c.cellCount[key], 1U);
+        // cell histogram: one atomic per distinct key of the wave (neighbouring
+        // particles of the canonical order mostly share a cell)
+        uint64_t todo = __ballot(1);
+        while (todo != 0ULL)
+        {
+            const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
+            const uint64_t same = __ballot(key == k0 ? 1 : 0);
+            if (key == k0)
+            {
+                if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(same)) - 1))
+                {
+                    atomicAdd(&c.cellCount[k0], static_cast<uint32_t>(__popcll(same)));
+                }
+                break;
+            }
+            todo &= ~same;
+        }
     }
 }
 
