@@ -62,24 +62,32 @@ def build(verbose: bool = True) -> Path:
     return LIB
 
 
-def build_cpp_test(verbose: bool = True) -> Path:
-    """The C++ interface test program (tests/cpp/facade_test.cpp), compiled by
-    the host compiler against include/ and linked to the library."""
+def _host_program(src: Path, out: Path, verbose: bool) -> Path:
     root = PKG.parent
-    out = LIB.parent / "facade_test"
     cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(root / "include"),
-           str(root / "tests" / "cpp" / "facade_test.cpp"), "-L", str(LIB.parent), "-lasph",
-           "-Wl,-rpath,$ORIGIN", "-o", str(out)]
+           str(src), "-L", str(LIB.parent), "-lasph", "-Wl,-rpath,$ORIGIN", "-o", str(out)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError(f"C++ interface test build failed:\n{r.stdout}\n{r.stderr}")
+        raise RuntimeError(f"build of {src.name} failed:\n{r.stdout}\n{r.stderr}")
     if r.stderr.strip():
-        print(f"[build] facade_test: {r.stderr[:4000]}", file=sys.stderr)
+        print(f"[build] {src.name}: {r.stderr[:4000]}", file=sys.stderr)
     if verbose:
         print(f"[build] {out}")
     return out
 
 
+def build_runner(verbose: bool = True) -> Path:
+    """The headless runner (runner/asph_run.cpp) over the C++ interface."""
+    return _host_program(PKG / "runner" / "asph_run.cpp", LIB.parent / "asph_run", verbose)
+
+
+def build_cpp_test(verbose: bool = True) -> Path:
+    """The C++ interface test program (tests/cpp/facade_test.cpp), compiled by
+    the host compiler against include/ and linked to the library."""
+    return _host_program(PKG.parent / "tests" / "cpp" / "facade_test.cpp", LIB.parent / "facade_test", verbose)
+
+
 if __name__ == "__main__":
     build()
+    build_runner()
     build_cpp_test()

@@ -75,6 +75,23 @@ def lattice(n: tuple[int, int, int], dx: float, origin: tuple[float, float, floa
     return out
 
 
+def application_grid(count: tuple[int, int, int], domain: abi.Domain) -> np.ndarray:
+    """The reference application's default particle grid (App.cpp:312-339):
+    count particles per axis filling the domain minus a 0.15 m margin, in FP32
+    as the application computes it, x outer and z innermost."""
+    f = np.float32
+    margin = f(0.15)
+    lo = np.array([domain.min.x, domain.min.y, domain.min.z], f)
+    hi = np.array([domain.max.x, domain.max.y, domain.max.z], f)
+    spacing = ((hi - lo) - f(2.0) * margin) / (np.asarray(count, f) - f(1.0))
+    base = lo + margin
+    i, j, k = np.meshgrid(np.arange(count[0]), np.arange(count[1]), np.arange(count[2]), indexing="ij")
+    idx = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(f)
+    out = np.zeros((idx.shape[0], 4), f)
+    out[:, :3] = base + spacing * idx
+    return out
+
+
 def physics(box_min, box_max, dx: float, gamma: float = 7.0, nu: float = 1e-6, h_ratio: float = 1.3,
             g: float = 9.81, c_height: float = 0.5) -> tuple[abi.Parameters, float]:
     """Parameters of a scene with lattice spacing dx, and its time step."""

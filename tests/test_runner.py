@@ -1,0 +1,143 @@
+"""The headless runner (adaptivesph_amd/runner/asph_run.cpp): configuration
+files in the reference application's JSON schema, its default scene, and a run
+through the C++ interface that matches the C ABI path bit for bit."""
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from adaptivesph_amd import abi, scene
+from adaptivesph_amd.simulation import Simulation
+
+RUN = Path(abi.PKG_DIR) / "lib" / "asph_run"
+
+# a configuration written the way the reference application's files are
+# (every field of Serializers.hpp), with walls within reach of the fluid
+CONFIG = {
+    "initialParameters": {"particleCount": {"x": 12, "y": 10, "z": 9}},
+    "simulationParameters": {
+        "domain": {"min": {"x": 0.0, "y": 0.0, "z": 0.0}, "max": {"x": 0.6, "y": 0.5, "z": 0.45}, "friction": 0.01},
+        "gravity": {"x": 0.0, "y": -9.81, "z": 0.0},
+        "speedOfSound": 20.0,
+        "restDensity": 1000.0,
+        "restitution": 0.5,
+        "viscosityConstant": 0.02,
+        "surfaceTensionConstant": 0.0,
+        "maxVelocity": 20.0,
+        "baseSmoothingRadius": 0.03,
+        "baseParticleRadius": 0.0125,
+        "baseParticleMass": 0.0125,
+        "threadsPerBlock": 256,
+    },
+    "refinementParameters": None,
+    "renderingParameters": {"renderBoundaryParticles": False},
+}
+
+
+def _run(*args, timeout=120):
+    if not RUN.exists():
+        raise FileNotFoundError(f"{RUN} is not built (run __graft_entry__.build())")
+    return subprocess.run([str(RUN), *map(str, args)], capture_output=True, text=True, timeout=timeout)
+
+
+def test_template_round_trip(tmp_path):
+    t = tmp_path / "template.json"
+    assert _run("--template", t).returncode == 0
+    r = _run(t, "--check")
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == t.read_text()
+    cfg = json.loads(t.read_text())
+    ref = cfg["refinementParameters"]
+    # defaults of RefinementParameters.cuh
+    assert ref["maxParticleCount"] == 1000000 and ref["cooldown"] == 10 and ref["criterionType"] == "velocity"
+    assert cfg["initialParameters"]["particleCount"] == {"x": 40, "y": 40, "z": 25}
+
+
+def test_reference_schema_parses(tmp_path):
+    c = json.loads(json.dumps(CONFIG))
+    c["refinementParameters"] = json.loads(_template(tmp_path))["refinementParameters"]
+    c["refinementParameters"]["criterionType"] = "vorticity"
+    c["refinementParameters"]["enabled"] = True
+    del c["refinementParameters"]["zone"]  # an extension; optional
+    p = tmp_path / "c.json"
+    p.write_text(json.dumps(c))
+    r = _run(p, "--check")
+    assert r.returncode == 0, r.stderr
+    out = json.loads(r.stdout)
+    assert out["refinementParameters"]["criterionType"] == "vorticity"
+    assert out["refinementParameters"]["enabled"] is True
+    assert out["simulationParameters"]["viscosityConstant"] == pytest.approx(0.02)
+    assert out["simulationParameters"]["taitGamma"] == 5  # reference exponent unless given
+
+
+def _template(tmp_path):
+    t = tmp_path / "t.json"
+    _run("--template", t)
+    return t.read_text()
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda c: c["simulationParameters"].pop("speedOfSound"), "speedOfSound"),
+    (lambda c: c.pop("initialParameters"), "initialParameters"),
+])
+def test_missing_required_fields_fail(tmp_path, edit, msg):
+    c = json.loads(json.dumps(CONFIG))
+    edit(c)
+    p = tmp_path / "bad.json"
+    p.write_text(json.dumps(c))
+    r = _run(p, "--check")
+    assert r.returncode == 1 and msg in r.stderr
+
+
+def test_application_grid_matches_reference_spacing():
+    d = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(0.6, 0.5, 0.45), 0.01)
+    g = scene.application_grid((12, 10, 9), d)
+    assert g.shape == (1080, 4)
+    assert g[0, :3].tolist() == pytest.approx([0.15, 0.15, 0.15])
+    assert g[-1, :3].tolist() == pytest.approx([0.45, 0.35, 0.30], abs=1e-6)
+    assert g[1, 2] > g[0, 2] and g[1, 0] == g[0, 0]  # z innermost
+
+
+@pytest.mark.gpu
+def test_runner_matches_c_abi(tmp_path):
+    cfgp = tmp_path / "c.json"
+    cfgp.write_text(json.dumps(CONFIG))
+    out = tmp_path / "out.bin"
+    steps = 30
+    r = _run(cfgp, "--steps", steps, "--dt", "0.0001", "--report-every", 10, "--dump", out)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [json.loads(x) for x in r.stdout.splitlines()]
+    assert lines[0]["step"] == 0 and lines[-1]["summary"]["steps"] == steps
+    raw = out.read_bytes()
+    n = int(np.frombuffer(raw[:4], np.uint32)[0])
+    f = np.frombuffer(raw[4:], np.float32)
+
+    p = abi.default_parameters()
+    sp = CONFIG["simulationParameters"]
+    p.domain = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(0.6, 0.5, 0.45), 0.01)
+    p.gravity = abi.Vec3(0.0, -9.81, 0.0)
+    for key, field in (("speedOfSound", "speed_of_sound"), ("restDensity", "rest_density"),
+                       ("restitution", "restitution"), ("viscosityConstant", "viscosity_constant"),
+                       ("surfaceTensionConstant", "surface_tension_constant"), ("maxVelocity", "max_velocity"),
+                       ("baseSmoothingRadius", "base_smoothing_radius"), ("baseParticleRadius", "base_particle_radius"),
+                       ("baseParticleMass", "base_particle_mass")):
+        setattr(p, field, sp[key])
+    pos = scene.application_grid((12, 10, 9), p.domain)
+    walls = scene.generate_boundary(p)
+    assert len(walls) > 0
+    sim = Simulation(p, pos, walls)
+    for _ in range(steps):
+        sim.update(np.float32(0.0001))
+    ref = sim.fluid()
+    sim.close()
+    assert n == len(pos)
+    x = f[: 4 * n].reshape(n, 4)
+    v = f[4 * n: 8 * n].reshape(n, 4)
+    rho = f[8 * n: 9 * n]
+    for name, a, b in (("positions", x, ref["positions"]), ("velocities", v, ref["velocities"]),
+                       ("densities", rho, ref["densities"])):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
+    # the walls are within 2h of the fluid here, so the boundary terms act
+    assert lines[-1]["density"]["max"] > 0
