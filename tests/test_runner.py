@@ -140,4 +140,6 @@ def test_runner_matches_c_abi(tmp_path):
                        ("densities", rho, ref["densities"])):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), name
     # the walls are within 2h of the fluid here, so the boundary terms act
-    assert lines[-1]["density"]["max"] > 0
+    reports = [x for x in lines if "step" in x]
+    assert [x["step"] for x in reports] == [0, 10, 20, 30]
+    assert reports[-1]["particles"] == n and reports[-1]["density"]["max"] > 0
