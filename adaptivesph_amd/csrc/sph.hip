@@ -591,18 +591,145 @@ __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint
     return c.list + (static_cast<size_t>(k / kChunk) * c.pCap + t) * kChunk + (k % kChunk);
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
+__device__ __forceinline__ float waveMinF(float v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = fminf(v, __shfl_xor(v, off, 64));
+    }
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+__device__ __forceinline__ float waveMaxF(float v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        v = fmaxf(v, __shfl_xor(v, off, 64));
+    }
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+
+// Wave-uniform walk with a lane-parallel candidate filter (the list builder's
+// walk). Cells are visited as in waveWalk (the union of the wave's candidate
+// cells, x, then y, then z innermost). A cell's slots are taken 64 at a time,
+// one per lane, and each is tested against the box of the wave's queries: a
+// candidate farther from that box than its own support 2h_j cannot be
+// accepted by any lane. The survivors (in ascending slot order) are
+// broadcast from LDS and every lane applies the exact test, so each lane still
+// sees exactly its reference neighbour sequence.
+//
+// The box distance is a lower bound of every lane's distance in FP32 as well:
+// each rounded difference, square and sum is monotonic in its operands and
+// uses the same association as the exact test; the margin is extra.
+template <class F>
+__device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid,
+                                                 const CellBox& b, const F4& q, F4* wstash, uint32_t lane,
+                                                 F&& append)
+{
+    ConstU32* off = (ConstU32*)(c.cellOff);
+    const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
+    const int uy0 = waveMinI(valid ? b.y0 : 0x7FFFFFFF);
+    const int uz0 = waveMinI(valid ? b.z0 : 0x7FFFFFFF);
+    const int ux1 = waveMaxI(valid ? b.x1 : -1);
+    const int uy1 = waveMaxI(valid ? b.y1 : -1);
+    const int uz1 = waveMaxI(valid ? b.z1 : -1);
+    const float qx0 = waveMinF(valid ? q.x : FLT_MAX);
+    const float qy0 = waveMinF(valid ? q.y : FLT_MAX);
+    const float qz0 = waveMinF(valid ? q.z : FLT_MAX);
+    const float qx1 = waveMaxF(valid ? q.x : -FLT_MAX);
+    const float qy1 = waveMaxF(valid ? q.y : -FLT_MAX);
+    const float qz1 = waveMaxF(valid ? q.z : -FLT_MAX);
+    const int gx = sc.gx;
+    const int gxgy = sc.gx * sc.gy;
+    const uint32_t base = sc.cellBase;
+    for (int x = ux0; x <= ux1; ++x)
+    {
+        for (int y = uy0; y <= uy1; ++y)
+        {
+            for (int z = uz0; z <= uz1; ++z)
+            {
+                const bool inbox =
+                    valid && x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1;
+                if (__ballot(inbox ? 1 : 0) == 0ULL)
+                {
+                    continue;
+                }
+                const uint32_t cell = base + static_cast<uint32_t>(x + y * gx + z * gxgy);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
+                const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+                for (uint32_t j0 = lo; j0 < hi; j0 += 64U)
+                {
+                    const uint32_t j = j0 + lane;
+                    const bool in = j < hi;
+                    const F4 o = c.s0[in ? j : hi - 1U];
+                    const float ex = fmaxf(fmaxf(qx0 - o.x, o.x - qx1), 0.0F);
+                    const float ey = fmaxf(fmaxf(qy0 - o.y, o.y - qy1), 0.0F);
+                    const float ez = fmaxf(fmaxf(qz0 - o.z, o.z - qz1), 0.0F);
+                    const float d2b = (ex * ex + ey * ey) + ez * ez;
+                    uint64_t m = __ballot((in && d2b <= o.w * 1.0001F) ? 1 : 0);
+                    if (m == 0ULL)
+                    {
+                        continue;
+                    }
+                    wstash[lane] = o;
+                    while (m != 0ULL)
+                    {
+                        // up to four survivors per round: their broadcasts are
+                        // read together, then tested in slot order
+                        uint32_t k[4];
+                        bool has[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4U; ++u)
+                        {
+                            has[u] = m != 0ULL;
+                            k[u] = has[u] ? static_cast<uint32_t>(__builtin_ctzll(m)) : 0U;
+                            m &= m - 1ULL;
+                        }
+                        F4 oc[4];
+#pragma unroll
+                        for (uint32_t u = 0; u < 4U; ++u)
+                        {
+                            oc[u] = wstash[k[u]];
+                        }
+#pragma unroll
+                        for (uint32_t u = 0; u < 4U; ++u)
+                        {
+                            if (has[u])
+                            {
+                                const float dx = oc[u].x - q.x;
+                                const float dy = oc[u].y - q.y;
+                                const float dz = oc[u].z - q.z;
+                                const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                                if (inbox && d2 <= oc[u].w)
+                                {
+                                    append(j0 + k[u]);
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// List builder. BS (the block size) is a compile-time constant so that the
+// staging addresses fold into immediates.
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
 {
     // staging ring: kChunk entries per thread, entry-major ([e][thread]) so
-    // that the lanes' appends fall in distinct banks
-    extern __shared__ uint32_t stage[];
+    // that the lanes' appends fall in distinct banks; one broadcast slot per
+    // lane for the candidate filter
+    __shared__ uint32_t stage[kChunk * BS];
+    __shared__ F4 stash[BS];
     const uint32_t n = *c.nTotal;
     const uint32_t nPad = (n + 63U) & ~63U;
     const uint32_t K = c.listCap;
-    const size_t CS = chunkStride(c);
     const uint32_t tid = threadIdx.x;
-    const uint32_t bd = blockDim.x;
-    for (uint32_t tw = blockIdx.x * blockDim.x + threadIdx.x; tw < nPad; tw += gridDim.x * blockDim.x)
+    const uint32_t lane = tid & 63U;
+    F4* wstash = stash + (tid & ~63U);
+    for (uint32_t tw = blockIdx.x * BS + tid; tw < nPad; tw += gridDim.x * BS)
     {
         const bool valid = tw < n;
         const uint32_t t = valid ? tw : n - 1U;
@@ -616,27 +743,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
         uint32_t cnt = 0;
-        uint32_t* lbase = c.list + static_cast<size_t>(t) * kChunk;
-        // writes the staged chunk `ch` of this thread as one 16-byte segment
-        auto flush = [&](uint32_t ch) {
-            *reinterpret_cast<uint4*>(lbase + ch * CS) =
-                uint4 {stage[tid], stage[bd + tid], stage[2U * bd + tid], stage[3U * bd + tid]};
+        // the thread's next chunk: chunk k of slot t is the 16 bytes at
+        // list + (k * pCap + t) * 4 entries
+        uint4* cptr = reinterpret_cast<uint4*>(c.list) + t;
+        const uint32_t cstep = c.pCap;
+        auto flush = [&]() {
+            *cptr = uint4 {stage[tid], stage[BS + tid], stage[2U * BS + tid], stage[3U * BS + tid]};
+            cptr += cstep;
         };
         auto append = [&](uint32_t j) {
-            if (cnt < K)
+            const uint32_t slot = cnt & (kChunk - 1U);
+            stage[slot * BS + tid] = j;
+            if (slot == kChunk - 1U && cnt < K)
             {
-                const uint32_t slot = cnt % kChunk;
-                stage[slot * bd + tid] = j;
-                if (slot == kChunk - 1U)
-                {
-                    flush(cnt / kChunk);
-                }
+                flush();
             }
             ++cnt;
         };
         if (oneScene)
         {
-            waveWalk(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, append);
+            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, lane, append);
         }
         else if (valid)
         {
@@ -646,7 +772,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         {
             if (cnt <= K && (cnt % kChunk) != 0U)
             {
-                flush(cnt / kChunk);  // partial last chunk (entries past cnt are never read)
+                flush();  // partial last chunk (entries past cnt are never read)
             }
             c.ncount[t] = cnt;
             if (cnt > K)
@@ -1921,7 +2047,18 @@ __host__ __forceinline__ uint32_t overflowGrid(const Ctx& c)
 void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
     const uint32_t b = lc.blockNeighbor;
-    hipLaunchKernelGGL(kNeighbors, dim3(gridFor(c.nMax, b)), dim3(b), b * kChunk * sizeof(uint32_t), s, c);
+    if (b == 64U)
+    {
+        hipLaunchKernelGGL(kNeighbors<64>, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    }
+    else if (b == 128U)
+    {
+        hipLaunchKernelGGL(kNeighbors<128>, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    }
+    else
+    {
+        hipLaunchKernelGGL(kNeighbors<256>, dim3(gridFor(c.nMax, 256U)), dim3(256), 0, s, c);
+    }
 }
 
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
