@@ -614,17 +614,21 @@ __device__ __forceinline__ float waveMaxF(float v)
 // cells, x, then y, then z innermost). A cell's slots are taken 64 at a time,
 // one per lane, and each is tested against the box of the wave's queries: a
 // candidate farther from that box than its own support 2h_j cannot be
-// accepted by any lane. The survivors (in ascending slot order) are
-// broadcast from LDS and every lane applies the exact test, so each lane still
-// sees exactly its reference neighbour sequence.
+// accepted by any lane. The survivors are packed in ascending slot order into
+// the wave's LDS table (`wstash`, `wslot`), and every lane applies the exact
+// test to each of them, so each lane still sees exactly its reference
+// neighbour sequence. push(hit, j) is called for every survivor (hit: the
+// lane accepts slot j), drain() after every four.
 //
 // The box distance is a lower bound of every lane's distance in FP32 as well:
 // each rounded difference, square and sum is monotonic in its operands and
 // uses the same association as the exact test; the margin is extra.
-template <class F>
+constexpr uint32_t kStashPad = 4;  // table entries past 64 read (and ignored) by the last round
+
+template <class P, class D>
 __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid,
-                                                 const CellBox& b, const F4& q, F4* wstash, uint32_t lane,
-                                                 F&& append)
+                                                 const CellBox& b, const F4& q, F4* wstash, uint32_t* wslot,
+                                                 uint32_t lane, P&& push, D&& drain)
 {
     ConstU32* off = (ConstU32*)(c.cellOff);
     const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
@@ -666,46 +670,42 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
                     const float ey = fmaxf(fmaxf(qy0 - o.y, o.y - qy1), 0.0F);
                     const float ez = fmaxf(fmaxf(qz0 - o.z, o.z - qz1), 0.0F);
                     const float d2b = (ex * ex + ey * ey) + ez * ez;
-                    uint64_t m = __ballot((in && d2b <= o.w * 1.0001F) ? 1 : 0);
+                    const bool keep = in && d2b <= o.w * 1.0001F;
+                    const uint64_t m = __ballot(keep ? 1 : 0);
                     if (m == 0ULL)
                     {
                         continue;
                     }
-                    wstash[lane] = o;
-                    while (m != 0ULL)
+                    const uint32_t S = static_cast<uint32_t>(__builtin_popcountll(m));
+                    if (keep)
                     {
-                        // up to four survivors per round: their broadcasts are
-                        // read together, then tested in slot order
-                        uint32_t k[4];
-                        bool has[4];
-#pragma unroll
-                        for (uint32_t u = 0; u < 4U; ++u)
-                        {
-                            has[u] = m != 0ULL;
-                            k[u] = has[u] ? static_cast<uint32_t>(__builtin_ctzll(m)) : 0U;
-                            m &= m - 1ULL;
-                        }
+                        // rank among the survivors = survivors in lower lanes
+                        const uint32_t r = __builtin_amdgcn_mbcnt_hi(
+                            static_cast<uint32_t>(m >> 32U), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0U));
+                        wstash[r] = o;
+                        wslot[r] = j;
+                    }
+                    for (uint32_t r = 0; r < S; r += 4U)
+                    {
                         F4 oc[4];
+                        uint32_t jj[4];
 #pragma unroll
                         for (uint32_t u = 0; u < 4U; ++u)
                         {
-                            oc[u] = wstash[k[u]];
+                            oc[u] = wstash[r + u];
+                            jj[u] = wslot[r + u];
                         }
 #pragma unroll
                         for (uint32_t u = 0; u < 4U; ++u)
                         {
-                            if (has[u])
-                            {
-                                const float dx = oc[u].x - q.x;
-                                const float dy = oc[u].y - q.y;
-                                const float dz = oc[u].z - q.z;
-                                const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                                if (inbox && d2 <= oc[u].w)
-                                {
-                                    append(j0 + k[u]);
-                                }
-                            }
+                            const float dx = oc[u].x - q.x;
+                            const float dy = oc[u].y - q.y;
+                            const float dz = oc[u].z - q.z;
+                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                            // bitwise: every lane tests, no branch
+                            push((d2 <= oc[u].w) & inbox & (r + u < S), jj[u]);
                         }
+                        drain();
                     }
                 }
             }
@@ -718,17 +718,21 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
 template <uint32_t BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
 {
-    // staging ring: kChunk entries per thread, entry-major ([e][thread]) so
-    // that the lanes' appends fall in distinct banks; one broadcast slot per
-    // lane for the candidate filter
-    __shared__ uint32_t stage[kChunk * BS];
-    __shared__ F4 stash[BS];
+    // staging ring of kRing entries per thread, entry-major ([e][thread]) so
+    // that the lanes' writes fall in distinct banks; per wave, the survivor
+    // table of the candidate filter
+    constexpr uint32_t kRing = 2U * kChunk;
+    constexpr uint32_t kTable = 64U + kStashPad;
+    __shared__ uint32_t stage[kRing * BS];
+    __shared__ F4 stash[(BS / 64U) * kTable];
+    __shared__ uint32_t slots[(BS / 64U) * kTable];
     const uint32_t n = *c.nTotal;
     const uint32_t nPad = (n + 63U) & ~63U;
     const uint32_t K = c.listCap;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63U;
-    F4* wstash = stash + (tid & ~63U);
+    F4* wstash = stash + (tid / 64U) * kTable;
+    uint32_t* wslot = slots + (tid / 64U) * kTable;
     for (uint32_t tw = blockIdx.x * BS + tid; tw < nPad; tw += gridDim.x * BS)
     {
         const bool valid = tw < n;
@@ -742,37 +746,50 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         const float hi = c.shm[t].x;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
+        // entries [done, cnt) are staged at ring positions (done .. cnt-1) mod
+        // kRing; at most kRing - 1 are staged, so position cnt mod kRing is free
         uint32_t cnt = 0;
+        uint32_t done = 0;
         // the thread's next chunk: chunk k of slot t is the 16 bytes at
         // list + (k * pCap + t) * 4 entries
         uint4* cptr = reinterpret_cast<uint4*>(c.list) + t;
         const uint32_t cstep = c.pCap;
-        auto flush = [&]() {
-            *cptr = uint4 {stage[tid], stage[BS + tid], stage[2U * BS + tid], stage[3U * BS + tid]};
-            cptr += cstep;
+        auto push = [&](bool hit, uint32_t j) {
+            stage[(cnt & (kRing - 1U)) * BS + tid] = j;
+            cnt += hit ? 1U : 0U;
         };
-        auto append = [&](uint32_t j) {
-            const uint32_t slot = cnt & (kChunk - 1U);
-            stage[slot * BS + tid] = j;
-            if (slot == kChunk - 1U && cnt < K)
+        // writes the oldest staged chunk once it is full (called after at most
+        // kChunk pushes, so the ring never holds more than kRing - 1 entries)
+        auto drain = [&]() {
+            if (cnt - done >= kChunk)
             {
-                flush();
+                if (done < K)
+                {
+                    const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
+                    *cptr = uint4 {stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]};
+                    cptr += cstep;
+                }
+                done += kChunk;
             }
-            ++cnt;
         };
         if (oneScene)
         {
-            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, lane, append);
+            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain);
         }
         else if (valid)
         {
-            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) { append(j); });
+            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
+                push(true, j);
+                drain();
+            });
         }
         if (valid)
         {
-            if (cnt <= K && (cnt % kChunk) != 0U)
+            if (cnt <= K && cnt > done)
             {
-                flush();  // partial last chunk (entries past cnt are never read)
+                // partial last chunk (entries past cnt are never read)
+                const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
+                *cptr = uint4 {stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]};
             }
             c.ncount[t] = cnt;
             if (cnt > K)
