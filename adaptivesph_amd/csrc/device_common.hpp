@@ -28,6 +28,7 @@ constexpr uint32_t kSceneAdaptive = 2U;
 constexpr uint32_t kSceneFixCellStart = 4U;
 constexpr uint32_t kSceneBoundaryInside = 8U;  // some boundary particle lies in an interior cell
 constexpr uint32_t kSceneKernelMergedH = 16U;  // merged h with the kernel normalisation
+constexpr uint32_t kSceneUniformHM = 32U;      // every particle has h = h0 and m = m0 (plain scenes)
 
 // Read-only per-scene description (uploaded when the scene set changes).
 struct SceneDev

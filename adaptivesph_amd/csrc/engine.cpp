@@ -3,8 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <limits>
 #include <cstring>
+#include <limits>
 #include <numeric>
 
 namespace asph
@@ -185,10 +185,10 @@ uint32_t listCapacityRounded(uint32_t listCap)
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
-    // s0/s1/s2 48 + shm 16 + scene id 4 + neighbour count 4, group boxes 4,
+    // s0/s1/fa/fb 64 + shm 16 + scene id 4 + neighbour count 4, group boxes 4,
     // neighbour list 4 per entry (capacity rounded up to whole 4-entry
     // chunks), overflow queue 4, radix histogram and scan tiles < 1.
-    uint64_t perParticle = 64ULL + 16ULL + 72ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 4ULL + 1ULL;
+    uint64_t perParticle = 64ULL + 16ULL + 88ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 4ULL + 1ULL;
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
@@ -256,7 +256,8 @@ Engine::Engine(const EngineConfig& cfg)
         c.s0 = alloc<F4>(P);
         c.s1 = alloc<F4>(P);
         c.shm = alloc<F4>(P);
-        c.s2 = alloc<F4>(P);
+        c.fa = alloc<F4>(P);
+        c.fb = alloc<F4>(P);
         c.gbox0 = alloc<F4>(P / kGroup + 1);
         c.gbox1 = alloc<F4>(P / kGroup + 1);
         c.sscene = alloc<uint32_t>(P);
@@ -487,6 +488,8 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     hs.adaptive = adaptive;
     hs.cellRegion = static_cast<uint32_t>(cells + 1);
     hs.count0 = count;
+    hs.knownH = count;
+    hs.knownM = count;
     SceneDev& d = hs.dev;
     d.dmin[0] = p.domain.min.x;
     d.dmin[1] = p.domain.min.y;
@@ -526,7 +529,8 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     d.bCount = nb;
     d.flags = kSceneActive | (adaptive ? kSceneAdaptive : 0U) |
               ((p.flags & ASPH_FLAG_FIX_CELL_START) != 0U ? kSceneFixCellStart : 0U) |
-              ((p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? kSceneKernelMergedH : 0U);
+              ((p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? kSceneKernelMergedH : 0U) |
+              (adaptive ? 0U : kSceneUniformHM);  // refinement is the only writer of h and m
     d.tpb = p.threads_per_block;
     if (r != nullptr)
     {
@@ -1193,6 +1197,31 @@ void Engine::setFluid(uint32_t id, uint32_t count, const asph_vec4* pos, const a
     put(ctx_.m + o, m, sizeof(float) * count);
     st.count = count;
     hipCheck(hipMemcpy(ctx_.states + id, &st, sizeof(SceneState), hipMemcpyHostToDevice), "upload");
+    // a plain scene keeps the uniform-h/m fast path while every particle has
+    // exactly h0 and m0
+    auto known = [&](const float* a, float v, uint32_t prev) {
+        if (a == nullptr)
+        {
+            return prev;
+        }
+        for (uint32_t i = 0; i < count; ++i)
+        {
+            if (std::memcmp(&a[i], &v, sizeof(float)) != 0)
+            {
+                return i;
+            }
+        }
+        return std::max(prev, count);
+    };
+    hs.knownH = known(h, hs.dev.h0, hs.knownH);
+    hs.knownM = known(m, hs.dev.m0, hs.knownM);
+    const bool uniform = !hs.adaptive && hs.knownH >= count && hs.knownM >= count;
+    const uint32_t flags = uniform ? (hs.dev.flags | kSceneUniformHM) : (hs.dev.flags & ~kSceneUniformHM);
+    if (flags != hs.dev.flags)
+    {
+        hs.dev.flags = flags;
+        uploadScene(id);
+    }
 }
 
 asph_fluid_view Engine::view(uint32_t id)

@@ -142,6 +142,9 @@ private:
         bool adaptive = false;
         uint32_t frame = 0;  // host mirror of the device frame counter
         uint32_t count0 = 0;
+        // leading slots known to hold h = h0 (knownH) and m = m0 (knownM)
+        uint32_t knownH = 0;
+        uint32_t knownM = 0;
         std::vector<F4> boundarySorted;
         std::vector<int32_t> boundaryKeysSorted;
     };

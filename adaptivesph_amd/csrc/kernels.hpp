@@ -70,7 +70,8 @@ struct Ctx
     F4* s0;      // (x, y, z, (2h)^2)
     F4* s1;      // (vx, vy, vz, m)
     F4* shm;     // (h, m, kWCoef / h^3, kWDCoef / h^4)
-    F4* s2;      // (rho, p / rho^2, h, kWDCoef / h^4)
+    F4* fa;      // (x, y, z, p / rho^2), written with the density
+    F4* fb;      // (vx, vy, vz, rho), written with the density
     uint32_t* sscene;
     uint32_t* ncount;
     // particles whose neighbours overflow the list (sorted slots), and their

@@ -568,6 +568,10 @@ __device__ __forceinline__ uint32_t xcdBlock()
 #define ASPH_FORCE_BATCH 4
 #endif
 constexpr uint32_t kDensityBatch = ASPH_DENSITY_BATCH;  // a multiple of 4
+#ifndef ASPH_DENSITY_UNIFORM_BATCH
+#define ASPH_DENSITY_UNIFORM_BATCH 12
+#endif
+constexpr uint32_t kDensityUniformBatch = ASPH_DENSITY_UNIFORM_BATCH;  // a multiple of 4
 constexpr uint32_t kForceBatch = ASPH_FORCE_BATCH;      // a multiple of 4
 
 // -------------------------------------------------------------- density --
@@ -585,6 +589,22 @@ constexpr uint32_t kForceBatch = ASPH_FORCE_BATCH;      // a multiple of 4
 constexpr uint32_t kChunk = 4;
 
 __device__ __forceinline__ size_t chunkStride(const Ctx& c) { return static_cast<size_t>(c.pCap) * kChunk; }
+
+// The consumers stream the list through once each; non-temporal loads keep it
+// from evicting the particle records that they gather (the list stores stay
+// ordinary: non-temporal stores measured 3x slower in the list builder).
+typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 loadChunk(const uint32_t* p)
+{
+    const U32x4 v = __builtin_nontemporal_load(reinterpret_cast<const U32x4*>(p));
+    return uint4 {v.x, v.y, v.z, v.w};
+}
+
+__device__ __forceinline__ void storeChunk(uint4* p, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    *p = uint4 {a, b, c, d};
+}
 
 __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint32_t k)
 {
@@ -766,7 +786,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
                 if (done < K)
                 {
                     const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                    *cptr = uint4 {stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]};
+                    storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
                     cptr += cstep;
                 }
                 done += kChunk;
@@ -789,7 +809,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
             {
                 // partial last chunk (entries past cnt are never read)
                 const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                *cptr = uint4 {stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]};
+                storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
             }
             c.ncount[t] = cnt;
             if (cnt > K)
@@ -855,7 +875,8 @@ __device__ __forceinline__ float boundaryDensity(const Ctx& c, const SceneDev& s
 }
 
 // Adds the boundary contribution (launched unconditionally by the
-// reference), then Tait pressure; stores (rho, p / rho^2, h, kWDCoef / h^4).
+// reference), then Tait pressure; stores the force kernels' gather records
+// fa = (position, p / rho^2) and fb = (velocity, rho).
 __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, const CellBox& b, const F4& q,
                                               float hi, uint32_t t, float rho)
 {
@@ -866,7 +887,9 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
     }
     rho = rho + rhoB;
     const float p = tait(rho, sc);
-    c.s2[t] = F4 {rho, p / (rho * rho), hi, c.shm[t].w};
+    const F4 v = c.s1[t];
+    c.fa[t] = F4 {q.x, q.y, q.z, p / (rho * rho)};
+    c.fb[t] = F4 {v.x, v.y, v.z, rho};
 }
 
 // Phase 2 of computeDensities over the neighbour list, plus
@@ -890,6 +913,52 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
         float rho = 0.0F;
+        if ((sc.flags & kSceneUniformHM) != 0U)
+        {
+            // every neighbour has h0 and m0: the same sum with one gather
+            // (the position) per pair
+            const float h0 = sc.h0;
+            const float m0 = sc.m0;
+            const float c3 = coefOverH3(h0);
+            uint32_t e = 0;
+            for (; e + kDensityUniformBatch <= cnt; e += kDensityUniformBatch)
+            {
+                uint32_t j[kDensityUniformBatch];
+                float3 o[kDensityUniformBatch];
+#pragma unroll
+                for (uint32_t u = 0; u < kDensityUniformBatch; u += 4U)
+                {
+                    const uint4 v = loadChunk(listAt(c, t, e + u));
+                    j[u] = v.x;
+                    j[u + 1U] = v.y;
+                    j[u + 2U] = v.z;
+                    j[u + 3U] = v.w;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kDensityUniformBatch; ++u)
+                {
+                    o[u] = *reinterpret_cast<const float3*>(c.s0 + j[u]);
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < kDensityUniformBatch; ++u)
+                {
+                    const float dx = o[u].x - q.x;
+                    const float dy = o[u].y - q.y;
+                    const float dz = o[u].z - q.z;
+                    const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
+                    rho = rho + m0 * wendlandC(d, h0, c3);
+                }
+            }
+            for (; e < cnt; ++e)
+            {
+                const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, t, e));
+                const float dx = o.x - q.x;
+                const float dy = o.y - q.y;
+                const float dz = o.z - q.z;
+                rho = rho + m0 * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, c3);
+            }
+        }
+        else
         {
             uint32_t e = 0;
             for (; e + kDensityBatch <= cnt; e += kDensityBatch)
@@ -900,7 +969,7 @@ __global__ __launch_bounds__(256) void kDensity(Ctx c)
 #pragma unroll
                 for (uint32_t u = 0; u < kDensityBatch; u += 4U)
                 {
-                    const uint4 v = *reinterpret_cast<const uint4*>(listAt(c, t, e + u));
+                    const uint4 v = loadChunk(listAt(c, t, e + u));
                     j[u] = v.x;
                     j[u + 1U] = v.y;
                     j[u + 2U] = v.z;
@@ -966,11 +1035,13 @@ struct PairAcc
 // One fluid pair of computePressureAccelerations (Algorithm.cu:72-105),
 // computeViscosityAccelerations (:129-171) and
 // computeSurfaceTensionAccelerations (:192-218), accumulated separately.
-// o = s0[j], nj = s1[j] (velocity, mass), rj = s2[j] (rho, p/rho^2, h, kWDCoef/h^4).
-__device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& rj, const F4& q, const F4& vi,
-                                          float rhoi, float prri, float nuNum, float epsH2, bool doVisc, bool doST,
-                                          PairAcc& A)
+// fa = (position, p/rho^2) and fb = (velocity, rho) of neighbour j; mj, hj,
+// wdj = its mass, smoothing length and kWDCoef / h^4.
+__device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, float hj, float wdj, const F4& q,
+                                          const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
+                                          bool doST, PairAcc& A)
 {
+    const F4& o = fa;
     const float ox = q.x - o.x;
     const float oy = q.y - o.y;
     const float oz = q.z - o.z;
@@ -985,21 +1056,20 @@ __device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& r
         dy = oy / d;
         dz = oz / d;
     }
-    const float mj = nj.w;
-    const float pterm = prri + rj.y;
-    const float dw = wendlandGradC(d, rj.z, rj.w);
+    const float pterm = prri + fa.w;
+    const float dw = wendlandGradC(d, hj, wdj);
     A.apx = A.apx - ((mj * dx) * dw) * pterm;
     A.apy = A.apy - ((mj * dy) * dw) * pterm;
     A.apz = A.apz - ((mj * dz) * dw) * pterm;
     if (doVisc)
     {
-        const float vx = vi.x - nj.x;
-        const float vy = vi.y - nj.y;
-        const float vz = vi.z - nj.z;
+        const float vx = vi.x - fb.x;
+        const float vy = vi.y - fb.y;
+        const float vz = vi.z - fb.z;
         const float comp = dot3w(vx, vy, vz, ox, oy, oz);
         if (comp < 0.0F)
         {
-            const float nu = nuNum / (rhoi + rj.x);
+            const float nu = nuNum / (rhoi + fb.w);
             const float piv = ((-nu) * comp) / (d2 + epsH2);
             const float sv = (mj * piv) * dw;
             A.avx = A.avx - sv * dx;
@@ -1009,7 +1079,7 @@ __device__ __forceinline__ void forcePair(const F4& o, const F4& nj, const F4& r
     }
     if (doST)
     {
-        const float sw = mj * wendland(d, rj.z);
+        const float sw = mj * wendland(d, hj);
         A.asx = A.asx + sw * dx;
         A.asy = A.asy + sw * dy;
         A.asz = A.asz + sw * dz;
@@ -1023,10 +1093,11 @@ __device__ __forceinline__ PairAcc forceWalk(const Ctx& c, const SceneDev& sc, u
                                           bool doVisc, bool doST, uint32_t t)
 {
     PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
-    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4& o, float) {
+    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
         if (j != t)
         {
-            forcePair(o, c.s1[j], c.s2[j], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            const F4 hm = c.shm[j];
+            forcePair(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
         }
     });
     return A;
@@ -1039,7 +1110,8 @@ struct BoundaryAcc
 };
 
 // computeBoundaryPressureAcceleration and computeBoundaryFrictionAcceleration
-// (BoundaryAlgorithm.cu:53-162) for one particle; ri = s2[t].
+// (BoundaryAlgorithm.cu:53-162) for one particle; ri = (rho, p / rho^2, h,
+// kWDCoef / h^4) of the particle.
 __device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDev& sc, const CellBox& b, const F4& q,
                                                   const F4& vi, const F4& ri, float epsH2)
 {
@@ -1170,6 +1242,78 @@ __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, co
     }
 }
 
+// (rho, p / rho^2, h, kWDCoef / h^4) of sorted slot t
+__device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
+{
+    const F4 hm = c.shm[t];
+    return F4 {c.fb[t].w, c.fa[t].w, hm.x, hm.w};
+}
+
+// Fluid pair sums of slot t over its neighbour list, in list order. With
+// kUniform every neighbour has h0 and m0 (two gathers per pair), otherwise
+// the mass and smoothing length are gathered too.
+template <bool kUniform>
+__device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q,
+                                          const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
+                                          bool doST, PairAcc& A)
+{
+    const float m0 = sc.m0;
+    const float h0 = sc.h0;
+    const float wd0 = kUniform ? dcoefOverH4(h0) : 0.0F;
+    uint32_t e = 0;
+    for (; e + kForceBatch <= cnt; e += kForceBatch)
+    {
+        uint32_t j[kForceBatch];
+        F4 fa[kForceBatch];
+        F4 fb[kForceBatch];
+        F4 hm[kForceBatch];
+#pragma unroll
+        for (uint32_t u = 0; u < kForceBatch; u += 4U)
+        {
+            const uint4 v = loadChunk(listAt(c, t, e + u));
+            j[u] = v.x;
+            j[u + 1U] = v.y;
+            j[u + 2U] = v.z;
+            j[u + 3U] = v.w;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kForceBatch; ++u)
+        {
+            fa[u] = c.fa[j[u]];
+            fb[u] = c.fb[j[u]];
+            if (!kUniform)
+            {
+                hm[u] = c.shm[j[u]];
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kForceBatch; ++u)
+        {
+            if (j[u] != t)
+            {
+                forcePair(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x, kUniform ? wd0 : hm[u].w,
+                          q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            }
+        }
+    }
+    for (; e < cnt; ++e)
+    {
+        const uint32_t jj = *listAt(c, t, e);
+        if (jj != t)
+        {
+            if (kUniform)
+            {
+                forcePair(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            }
+            else
+            {
+                const F4 h1 = c.shm[jj];
+                forcePair(c.fa[jj], c.fb[jj], h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void kForce(Ctx c)
 {
     const uint32_t n = *c.nTotal;
@@ -1186,7 +1330,7 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         const SceneState& st = c.states[s];
         const F4 q = c.s0[t];
         const F4 vi = c.s1[t];
-        const F4 ri = c.s2[t];
+        const F4 ri = ownRecord(c, t);
         const float hi = ri.z;
         const float rhoi = ri.x;
         const float prri = ri.y;
@@ -1196,52 +1340,18 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
         PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
-        uint32_t e = 0;
-        for (; e + kForceBatch <= cnt; e += kForceBatch)
+        if ((sc.flags & kSceneUniformHM) != 0U)
         {
-            uint32_t j[kForceBatch];
-            F4 o[kForceBatch];
-            F4 nj[kForceBatch];
-            F4 rj[kForceBatch];
-#pragma unroll
-            for (uint32_t u = 0; u < kForceBatch; u += 4U)
-            {
-                const uint4 v = *reinterpret_cast<const uint4*>(listAt(c, t, e + u));
-                j[u] = v.x;
-                j[u + 1U] = v.y;
-                j[u + 2U] = v.z;
-                j[u + 3U] = v.w;
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kForceBatch; ++u)
-            {
-                o[u] = c.s0[j[u]];
-                nj[u] = c.s1[j[u]];
-                rj[u] = c.s2[j[u]];
-            }
-#pragma unroll
-            for (uint32_t u = 0; u < kForceBatch; ++u)
-            {
-                if (j[u] != t)
-                {
-                    forcePair(o[u], nj[u], rj[u], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-                }
-            }
+            forceList<true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
         }
-        for (; e < cnt; ++e)
+        else
         {
-            const uint32_t j0 = *listAt(c, t, e);
-            if (j0 != t)
-            {
-                forcePair(c.s0[j0], c.s1[j0], c.s2[j0], q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-            }
+            forceList<false>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
         }
         finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, t, A);
     }
 }
 
-// kForce for the particles whose neighbours overflow their list: the walk
-// again, in the same order.
 __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
 {
     const uint32_t m = *c.overflowCount;
@@ -1253,7 +1363,7 @@ __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
         const SceneState& st = c.states[s];
         const F4 q = c.s0[t];
         const F4 vi = c.s1[t];
-        const F4 ri = c.s2[t];
+        const F4 ri = ownRecord(c, t);
         const float hi = ri.z;
         const bool doVisc = sc.alpha != 0.0F;
         const bool doST = sc.sigma != 0.0F;

@@ -141,7 +141,10 @@ typedef struct asph_boundary_particle {
 /* FluidParticlesData (Simulation.cuh:22-33): device pointers, canonical order
    (index i is the particle the caller created at index i, renumbered only by
    refinement exactly as the reference renumbers). Valid until the next call
-   that changes the simulation. */
+   that changes the simulation. Read-only: the library keeps facts about the
+   state it owns (a plain simulation's smoothing lengths and masses all equal
+   the base values, which its density loop relies on); asph_debug_set_fluid is
+   the writer. */
 typedef struct asph_fluid_view {
     asph_vec4* positions;
     asph_vec4* velocities;
