@@ -573,6 +573,10 @@ constexpr uint32_t kDensityBatch = ASPH_DENSITY_BATCH;  // a multiple of 4
 #endif
 constexpr uint32_t kDensityUniformBatch = ASPH_DENSITY_UNIFORM_BATCH;  // a multiple of 4
 constexpr uint32_t kForceBatch = ASPH_FORCE_BATCH;      // a multiple of 4
+#ifndef ASPH_FORCE_UNIFORM_BATCH
+#define ASPH_FORCE_UNIFORM_BATCH 8
+#endif
+constexpr uint32_t kForceUniformBatch = ASPH_FORCE_UNIFORM_BATCH;  // a multiple of 4
 
 // -------------------------------------------------------------- density --
 // computeDensities (Algorithm.cu:17-54) and computeBoundaryDensityContribution
@@ -1260,15 +1264,16 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     const float m0 = sc.m0;
     const float h0 = sc.h0;
     const float wd0 = kUniform ? dcoefOverH4(h0) : 0.0F;
+    constexpr uint32_t B = kUniform ? kForceUniformBatch : kForceBatch;
     uint32_t e = 0;
-    for (; e + kForceBatch <= cnt; e += kForceBatch)
+    for (; e + B <= cnt; e += B)
     {
-        uint32_t j[kForceBatch];
-        F4 fa[kForceBatch];
-        F4 fb[kForceBatch];
-        F4 hm[kForceBatch];
+        uint32_t j[B];
+        F4 fa[B];
+        F4 fb[B];
+        F4 hm[B];
 #pragma unroll
-        for (uint32_t u = 0; u < kForceBatch; u += 4U)
+        for (uint32_t u = 0; u < B; u += 4U)
         {
             const uint4 v = loadChunk(listAt(c, t, e + u));
             j[u] = v.x;
@@ -1277,7 +1282,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
             j[u + 3U] = v.w;
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kForceBatch; ++u)
+        for (uint32_t u = 0; u < B; ++u)
         {
             fa[u] = c.fa[j[u]];
             fb[u] = c.fb[j[u]];
@@ -1287,7 +1292,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
             }
         }
 #pragma unroll
-        for (uint32_t u = 0; u < kForceBatch; ++u)
+        for (uint32_t u = 0; u < B; ++u)
         {
             if (j[u] != t)
             {
