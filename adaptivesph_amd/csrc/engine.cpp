@@ -182,6 +182,10 @@ uint32_t listCapacityRounded(uint32_t listCap)
     return ((k + 3U) / 4U) * 4U;
 }
 
+// Boundary pool ranges are whole culling groups, so a group never holds two
+// scenes' particles.
+uint64_t boundarySlots(uint64_t nb) { return (nb + kBoundaryGroup - 1) / kBoundaryGroup * kBoundaryGroup; }
+
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
@@ -195,9 +199,9 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
         // 16, compaction copies 64, merge list capture 4
         perParticle += 24ULL + 16ULL + 64ULL + 4ULL;
     }
-    // boundary slot: position + psi 16, viscosity 4; cell: count, offset and
-    // boundary offset 12, boundary box 32.
-    return perParticle * capacity + 20ULL * boundaryCount + 44ULL * (cells + 1ULL);
+    // boundary slot (whole groups of 16): position + psi 16, viscosity 4,
+    // group box 2; cell: count, offset and boundary offset 12, boundary box 32.
+    return perParticle * capacity + 22ULL * boundarySlots(boundaryCount) + 44ULL * (cells + 1ULL);
 }
 
 // --------------------------------------------------------------- engine --
@@ -214,7 +218,11 @@ T* Engine::alloc(size_t n)
 }
 
 Engine::Engine(const EngineConfig& cfg)
-    : cfg_(cfg), pAlloc_(cfg.particleCapacity), bAlloc_(cfg.boundaryCapacity), cAlloc_(cfg.cellCapacity)
+    : cfg_(cfg), pAlloc_(cfg.particleCapacity),
+      // room for the rounding of every scene's range to whole groups, so that
+      // scenes whose counts sum to boundaryCapacity fit as before
+      bAlloc_(boundarySlots(cfg.boundaryCapacity) + static_cast<uint64_t>(kBoundaryGroup) * cfg.sceneSlots),
+      cAlloc_(cfg.cellCapacity)
 {
     if (cfg_.sceneSlots == 0)
     {
@@ -224,7 +232,7 @@ Engine::Engine(const EngineConfig& cfg)
     try
     {
         const uint32_t P = std::max<uint32_t>(cfg_.particleCapacity, 1);
-        const uint32_t B = std::max<uint32_t>(cfg_.boundaryCapacity, 1);
+        const uint32_t B = static_cast<uint32_t>(std::max<uint64_t>(bAlloc_.capacity(), 1));
         const uint32_t C = std::max<uint32_t>(cfg_.cellCapacity, 2);
         const uint32_t S = cfg_.sceneSlots;
         Ctx& c = ctx_;
@@ -272,6 +280,8 @@ Engine::Engine(const EngineConfig& cfg)
         c.bcellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);
         c.bbox0 = alloc<F4>(static_cast<size_t>(C) + 1);
         c.bbox1 = alloc<F4>(static_cast<size_t>(C) + 1);
+        c.bgbox0 = alloc<F4>(B / kBoundaryGroup + 1);
+        c.bgbox1 = alloc<F4>(B / kBoundaryGroup + 1);
         if (cfg_.adaptive)
         {
             c.ckeysA = alloc<uint64_t>(P);
@@ -468,7 +478,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     {
         throw Error(ASPH_ERR_CAPACITY, "particle pool exhausted");
     }
-    const uint64_t bOff = bAlloc_.alloc(nb);
+    const uint64_t bOff = bAlloc_.alloc(boundarySlots(nb));
     if (bOff == UINT64_MAX)
     {
         pAlloc_.release(pOff, capacity);
@@ -478,7 +488,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     if (cBase == UINT64_MAX)
     {
         pAlloc_.release(pOff, capacity);
-        bAlloc_.release(bOff, nb);
+        bAlloc_.release(bOff, boundarySlots(nb));
         throw Error(ASPH_ERR_CAPACITY, "cell pool exhausted");
     }
     synchronize();
@@ -693,6 +703,29 @@ void Engine::uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32
     }
     hipCheck(hipMemcpy(ctx_.bbox0 + d.cellBase, lo.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
     hipCheck(hipMemcpy(ctx_.bbox1 + d.cellBase, hi.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
+    // group boxes over the sorted boundary slots: a cell at an edge or a
+    // corner of the domain collects several walls, so its box reaches into the
+    // fluid, while a group of consecutive slots is one wall's strip
+    if (nb > 0)
+    {
+        const uint32_t g0 = d.bOff / kBoundaryGroup;
+        const uint32_t ng = (nb + kBoundaryGroup - 1) / kBoundaryGroup;
+        std::vector<F4> glo(ng, F4 {inf, inf, inf, 0.0F});
+        std::vector<F4> ghi(ng, F4 {-inf, -inf, -inf, 0.0F});
+        for (uint32_t k = 0; k < nb; ++k)
+        {
+            const uint32_t g = k / kBoundaryGroup;  // bOff is a multiple of the group size
+            const F4& p = s0[k];
+            glo[g].x = std::min(glo[g].x, p.x);
+            glo[g].y = std::min(glo[g].y, p.y);
+            glo[g].z = std::min(glo[g].z, p.z);
+            ghi[g].x = std::max(ghi[g].x, p.x);
+            ghi[g].y = std::max(ghi[g].y, p.y);
+            ghi[g].z = std::max(ghi[g].z, p.z);
+        }
+        hipCheck(hipMemcpy(ctx_.bgbox0 + g0, glo.data(), sizeof(F4) * ng, hipMemcpyHostToDevice), "upload");
+        hipCheck(hipMemcpy(ctx_.bgbox1 + g0, ghi.data(), sizeof(F4) * ng, hipMemcpyHostToDevice), "upload");
+    }
 }
 
 void Engine::uploadScene(uint32_t id)
@@ -759,7 +792,7 @@ void Engine::removeScene(uint32_t id)
     synchronize();
     HostScene& hs = scenes_[id];
     pAlloc_.release(hs.dev.pOff, hs.dev.pCap);
-    bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
+    bAlloc_.release(hs.dev.bOff, boundarySlots(hs.dev.bCount));
     cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
     hs.active = false;
     hs.dev.flags = 0;
@@ -805,7 +838,7 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     const GridGeom g = makeGrid(dom, hs.dev.h0);
     const uint64_t cells = g.cells();
     cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
-    bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
+    bAlloc_.release(hs.dev.bOff, boundarySlots(hs.dev.bCount));
     // NeighborGrid::updateBoundarySize reallocates the cell arrays for the new
     // domain (NeighborGrid.cu:51-85); the pools grow when the new grid or wall
     // lattice does not fit.
@@ -815,11 +848,11 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
         growCells(cAlloc_.capacity() + cells + 1);
         cBase = cAlloc_.alloc(cells + 1);
     }
-    uint64_t bOff = bAlloc_.alloc(nb);
+    uint64_t bOff = bAlloc_.alloc(boundarySlots(nb));
     if (bOff == UINT64_MAX)
     {
-        growBoundary(bAlloc_.capacity() + nb);
-        bOff = bAlloc_.alloc(nb);
+        growBoundary(bAlloc_.capacity() + boundarySlots(nb));
+        bOff = bAlloc_.alloc(boundarySlots(nb));
     }
     if (cBase == UINT64_MAX || bOff == UINT64_MAX)
     {
@@ -898,6 +931,8 @@ void Engine::growBoundary(uint64_t newCap)
     const size_t B = std::max<size_t>(newCap, oldB);
     regrow(ctx_.bs0, oldB, B);
     regrow(ctx_.bvisc, oldB, B);
+    regrow(ctx_.bgbox0, oldB / kBoundaryGroup + 1, B / kBoundaryGroup + 1);
+    regrow(ctx_.bgbox1, oldB / kBoundaryGroup + 1, B / kBoundaryGroup + 1);
     bAlloc_.grow(B);
     invalidateGraphs();
 }

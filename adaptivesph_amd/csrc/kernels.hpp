@@ -92,6 +92,8 @@ struct Ctx
     // (min xyz, max xyz; empty cells hold an inverted box)
     F4* bbox0;
     F4* bbox1;
+    F4* bgbox0;  // per kBoundaryGroup boundary slots: box of their positions
+    F4* bgbox1;
 
     // refinement scratch (dense-indexed)
     uint64_t* ckeysA;

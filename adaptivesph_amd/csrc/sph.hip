@@ -425,120 +425,10 @@ __device__ __forceinline__ int waveMaxI(int v)
 }
 
 
-// Wave-uniform walk of the fluid grid for the 64 queries of a wave (all of one
-// scene). Every lane visits the union of the wave's candidate cells in the
-// reference's order (x, then y, then z innermost; slots ascending) and accepts
-// a candidate only if its cell lies in the lane's own candidate box and
-// d^2 <= (2 h_j)^2, so each lane sees exactly its reference neighbour
-// sequence. Candidates are read with wave-uniform addresses, and groups of
-// kGroup slots out of reach of every lane whose box holds the cell are
-// skipped.
-// Read-only views in the constant address space: loads with wave-uniform
-// addresses then go through the scalar cache (the arrays are not written by
-// the kernels that read them this way).
-typedef const __attribute__((address_space(4))) F4 ConstF4;
+// Read-only view in the constant address space: loads with wave-uniform
+// addresses then go through the scalar cache (the array is not written by the
+// kernels that read it this way).
 typedef const __attribute__((address_space(4))) uint32_t ConstU32;
-
-__device__ __forceinline__ F4 ldc(ConstF4* p, uint32_t i) { return F4 {p[i].x, p[i].y, p[i].z, p[i].w}; }
-
-template <class F>
-__device__ __forceinline__ void waveWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid, const CellBox& b,
-                                         const F4& q, F&& append)
-{
-    ConstF4* s0 = (ConstF4*)(c.s0);
-    ConstF4* gb0 = (ConstF4*)(c.gbox0);
-    ConstF4* gb1 = (ConstF4*)(c.gbox1);
-    ConstU32* off = (ConstU32*)(c.cellOff);
-    const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
-    const int uy0 = waveMinI(valid ? b.y0 : 0x7FFFFFFF);
-    const int uz0 = waveMinI(valid ? b.z0 : 0x7FFFFFFF);
-    const int ux1 = waveMaxI(valid ? b.x1 : -1);
-    const int uy1 = waveMaxI(valid ? b.y1 : -1);
-    const int uz1 = waveMaxI(valid ? b.z1 : -1);
-    const int gx = sc.gx;
-    const int gxgy = sc.gx * sc.gy;
-    const uint32_t base = sc.cellBase;
-    for (int x = ux0; x <= ux1; ++x)
-    {
-        for (int y = uy0; y <= uy1; ++y)
-        {
-            for (int z = uz0; z <= uz1; ++z)
-            {
-                const bool inbox =
-                    valid && x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1;
-                if (__ballot(inbox ? 1 : 0) == 0ULL)
-                {
-                    continue;
-                }
-                const uint32_t cell = base + static_cast<uint32_t>(x + y * gx + z * gxgy);
-                const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
-                const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
-                if (lo >= hi)
-                {
-                    continue;
-                }
-                const uint32_t g1 = (hi - 1U) / kGroup;
-                for (uint32_t g = lo / kGroup; g <= g1; ++g)
-                {
-                    const F4 b0 = ldc(gb0, g);
-                    const F4 b1 = ldc(gb1, g);
-                    // skip the group unless some lane whose box holds this cell
-                    // may reach a member (distance from the lane's query to the
-                    // group box against the group's largest support, with a
-                    // margin far above FP32 rounding)
-                    const float ex = fmaxf(fmaxf(b0.x - q.x, q.x - b1.x), 0.0F);
-                    const float ey = fmaxf(fmaxf(b0.y - q.y, q.y - b1.y), 0.0F);
-                    const float ez = fmaxf(fmaxf(b0.z - q.z, q.z - b1.z), 0.0F);
-                    const float d2g = (ex * ex + ey * ey) + ez * ez;
-                    if (__ballot((inbox && d2g <= b0.w * 1.0001F) ? 1 : 0) == 0ULL)
-                    {
-                        continue;
-                    }
-                    const uint32_t gb = g * kGroup;
-                    if (gb >= lo && gb + kGroup <= hi)
-                    {
-                        // whole group inside the cell range
-                        F4 o[kGroup];
-#pragma unroll
-                        for (uint32_t u = 0; u < kGroup; ++u)
-                        {
-                            o[u] = ldc(s0, gb + u);
-                        }
-#pragma unroll
-                        for (uint32_t u = 0; u < kGroup; ++u)
-                        {
-                            const float dx = o[u].x - q.x;
-                            const float dy = o[u].y - q.y;
-                            const float dz = o[u].z - q.z;
-                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            if (inbox && d2 <= o[u].w)
-                            {
-                                append(gb + u);
-                            }
-                        }
-                    }
-                    else
-                    {
-                        const uint32_t jlo = max(lo, gb);
-                        const uint32_t jhi = min(hi, gb + kGroup);
-                        for (uint32_t j = jlo; j < jhi; ++j)
-                        {
-                            const F4 o = ldc(s0, j);
-                            const float dx = o.x - q.x;
-                            const float dy = o.y - q.y;
-                            const float dz = o.z - q.z;
-                            const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            if (inbox && d2 <= o.w)
-                            {
-                                append(j);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
-}
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
 // XCD i mod 8), and each XCD has its own L2. The neighbour-list consumers
@@ -592,8 +482,6 @@ constexpr uint32_t kForceUniformBatch = ASPH_FORCE_UNIFORM_BATCH;  // a multiple
 // coalesced), four entries per 16-byte load.
 constexpr uint32_t kChunk = 4;
 
-__device__ __forceinline__ size_t chunkStride(const Ctx& c) { return static_cast<size_t>(c.pCap) * kChunk; }
-
 // The consumers stream the list through once each; non-temporal loads keep it
 // from evicting the particle records that they gather (the list stores stay
 // ordinary: non-temporal stores measured 3x slower in the list builder).
@@ -634,8 +522,10 @@ __device__ __forceinline__ float waveMaxF(float v)
 }
 
 // Wave-uniform walk with a lane-parallel candidate filter (the list builder's
-// walk). Cells are visited as in waveWalk (the union of the wave's candidate
-// cells, x, then y, then z innermost). A cell's slots are taken 64 at a time,
+// walk). The 64 lanes of a wave (all of one scene) visit the union of their
+// candidate cells in the reference's order (x, then y, then z innermost); a
+// lane accepts a candidate only if its cell lies in the lane's own candidate
+// box and d^2 <= (2 h_j)^2. A cell's slots are taken 64 at a time,
 // one per lane, and each is tested against the box of the wave's queries: a
 // candidate farther from that box than its own support 2h_j cannot be
 // accepted by any lane. The survivors are packed in ascending slot order into
@@ -860,16 +750,24 @@ __device__ __forceinline__ float boundaryDensity(const Ctx& c, const SceneDev& s
                 {
                     continue;
                 }
-                for (uint32_t j = lo; j < hiS; ++j)
+                for (uint32_t g = lo / kBoundaryGroup; g <= (hiS - 1U) / kBoundaryGroup; ++g)
                 {
-                    const F4 o = c.bs0[j];
-                    const float dx = o.x - q.x;
-                    const float dy = o.y - q.y;
-                    const float dz = o.z - q.z;
-                    const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                    if (d2 <= r2)
+                    if (!boxMayReach(c.bgbox0[g], c.bgbox1[g], q, r2))
                     {
-                        rhoB = rhoB + o.w * wendland(sqrtf(d2), hi);
+                        continue;
+                    }
+                    const uint32_t jhi = min(hiS, (g + 1U) * kBoundaryGroup);
+                    for (uint32_t j = max(lo, g * kBoundaryGroup); j < jhi; ++j)
+                    {
+                        const F4 o = c.bs0[j];
+                        const float dx = o.x - q.x;
+                        const float dy = o.y - q.y;
+                        const float dz = o.z - q.z;
+                        const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                        if (d2 <= r2)
+                        {
+                            rhoB = rhoB + o.w * wendland(sqrtf(d2), hi);
+                        }
                     }
                 }
             }
@@ -1139,42 +1037,50 @@ __device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDe
                 {
                     continue;
                 }
-                for (uint32_t j = lo; j < hiS; ++j)
+                for (uint32_t g = lo / kBoundaryGroup; g <= (hiS - 1U) / kBoundaryGroup; ++g)
                 {
-                    const F4 o = c.bs0[j];
-                    const float ox = q.x - o.x;
-                    const float oy = q.y - o.y;
-                    const float oz = q.z - o.z;
-                    const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
-                    if (d2 > r2)
+                    if (!boxMayReach(c.bgbox0[g], c.bgbox1[g], q, r2))
                     {
                         continue;
                     }
-                    const float d = sqrtf(d2);
-                    float dx = 0.0F;
-                    float dy = 1.0F;
-                    float dz = 0.0F;
-                    if (d > 0.0F)
+                    const uint32_t jhi = min(hiS, (g + 1U) * kBoundaryGroup);
+                    for (uint32_t j = max(lo, g * kBoundaryGroup); j < jhi; ++j)
                     {
-                        dx = ox / d;
-                        dy = oy / d;
-                        dz = oz / d;
-                    }
-                    const float psi = o.w;
-                    const float dw = wendlandGradC(d, hi, ri.w);
-                    const float sp = (psi * prri) * dw;
-                    B.bpx = B.bpx - sp * dx;
-                    B.bpy = B.bpy - sp * dy;
-                    B.bpz = B.bpz - sp * dz;
-                    const float comp = dot3w(vi.x, vi.y, vi.z, ox, oy, oz);
-                    if (comp < 0.0F)
-                    {
-                        const float f = c.bvisc[j];
-                        const float nu = ((f * hi) * sc.c0) / (2.0F * rhoi);
-                        const float piv = ((-nu) * comp) / (d2 + epsH2);
-                        B.bfx = B.bfx - ((dx * psi) * piv) * dw;
-                        B.bfy = B.bfy - ((dy * psi) * piv) * dw;
-                        B.bfz = B.bfz - ((dz * psi) * piv) * dw;
+                        const F4 o = c.bs0[j];
+                        const float ox = q.x - o.x;
+                        const float oy = q.y - o.y;
+                        const float oz = q.z - o.z;
+                        const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
+                        if (d2 > r2)
+                        {
+                            continue;
+                        }
+                        const float d = sqrtf(d2);
+                        float dx = 0.0F;
+                        float dy = 1.0F;
+                        float dz = 0.0F;
+                        if (d > 0.0F)
+                        {
+                            dx = ox / d;
+                            dy = oy / d;
+                            dz = oz / d;
+                        }
+                        const float psi = o.w;
+                        const float dw = wendlandGradC(d, hi, ri.w);
+                        const float sp = (psi * prri) * dw;
+                        B.bpx = B.bpx - sp * dx;
+                        B.bpy = B.bpy - sp * dy;
+                        B.bpz = B.bpz - sp * dz;
+                        const float comp = dot3w(vi.x, vi.y, vi.z, ox, oy, oz);
+                        if (comp < 0.0F)
+                        {
+                            const float f = c.bvisc[j];
+                            const float nu = ((f * hi) * sc.c0) / (2.0F * rhoi);
+                            const float piv = ((-nu) * comp) / (d2 + epsH2);
+                            B.bfx = B.bfx - ((dx * psi) * piv) * dw;
+                            B.bfy = B.bfy - ((dy * psi) * piv) * dw;
+                            B.bfz = B.bfz - ((dz * psi) * piv) * dw;
+                        }
                     }
                 }
             }
@@ -1403,6 +1309,31 @@ __device__ __forceinline__ void walkCanonical(const Ctx& c, const SceneDev& sc, 
                 for (uint32_t slot = lo; slot < hiS; ++slot)
                 {
                     f(c.sortedVals[slot]);
+                }
+            }
+        }
+    }
+}
+
+// walkCanonical's cells and order, passing the sorted slot.
+template <class F>
+__device__ __forceinline__ void walkSlots(const Ctx& c, const SceneDev& sc, uint32_t lost, float px, float py,
+                                          float pz, float R, F&& f)
+{
+    const CellBox b = cellBox(sc, px, py, pz, R);
+    const int gxgy = sc.gx * sc.gy;
+    for (int x = b.x0; x <= b.x1; ++x)
+    {
+        for (int y = b.y0; y <= b.y1; ++y)
+        {
+            for (int z = b.z0; z <= b.z1; ++z)
+            {
+                const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
+                const uint32_t lo = max(c.cellOff[cell], lost);
+                const uint32_t hiS = c.cellOff[cell + 1U];
+                for (uint32_t slot = lo; slot < hiS; ++slot)
+                {
+                    f(slot);
                 }
             }
         }
@@ -1694,27 +1625,33 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
         const uint32_t i = sc.pOff + id;
         const F4 q = c.pos[i];
         const float hh = c.h[i];
+        const float hh2 = hh * hh;
         const float massLimit = sc.maxMassRatio * sc.m0;
         float closest = FLT_MAX;
-        uint32_t best = kNoCandidate;
-        walkCanonical(c, sc, st.lostEnd, q.x, q.y, q.z, hh, [&](uint32_t j) {
-            if (j == i)
-            {
-                return;
-            }
-            const F4 pj = c.pos[j];
-            const float ox = pj.x - q.x;
-            const float oy = pj.y - q.y;
-            const float oz = pj.z - q.z;
+        uint32_t bestSlot = UINT32_MAX;
+        // The walk reads the sorted shadows, which hold the candidates'
+        // current positions, smoothing lengths and masses (nothing changes
+        // them between the gather and the merge), in the reference's order.
+        // d2 < hmin^2 implies d2 < h_i^2, so the shadow's h and m are read only
+        // past that test, and the candidate's identity only on acceptance.
+        walkSlots(c, sc, st.lostEnd, q.x, q.y, q.z, hh, [&](uint32_t slot) {
+            const F4 o = c.s0[slot];
+            const float ox = o.x - q.x;
+            const float oy = o.y - q.y;
+            const float oz = o.z - q.z;
             const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
-            const float hmin = gmin(hh, c.h[j]);
-            if (d2 < hmin * hmin && d2 < closest && c.m[j] < massLimit)
+            if (d2 < hh2)
             {
-                closest = d2;
-                best = j - sc.pOff;
+                const F4 hm = c.shm[slot];
+                const float hmin = gmin(hh, hm.x);
+                if (d2 < hmin * hmin && d2 < closest && hm.y < massLimit && c.sortedVals[slot] != i)
+                {
+                    closest = d2;
+                    bestSlot = slot;
+                }
             }
         });
-        c.cand[st.denseOff + id] = best;
+        c.cand[st.denseOff + id] = bestSlot == UINT32_MAX ? kNoCandidate : c.sortedVals[bestSlot] - sc.pOff;
     }
 }
 
