@@ -21,7 +21,7 @@ constexpr uint32_t kWave = 64;
 #define ASPH_GROUP 16
 #endif
 constexpr uint32_t kGroup = ASPH_GROUP;  // sorted slots per culling box (a power of two, at most 64)
-constexpr uint32_t kBoundaryGroup = 16;  // boundary slots per culling box; boundary ranges are whole groups
+constexpr uint32_t kBoundaryGroup = 16;  // boundary slots per culling box (chunks of a cell's range)
 
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;

@@ -182,10 +182,6 @@ uint32_t listCapacityRounded(uint32_t listCap)
     return ((k + 3U) / 4U) * 4U;
 }
 
-// Boundary pool ranges are whole culling groups, so a group never holds two
-// scenes' particles.
-uint64_t boundarySlots(uint64_t nb) { return (nb + kBoundaryGroup - 1) / kBoundaryGroup * kBoundaryGroup; }
-
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)
 {
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
@@ -199,9 +195,9 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
         // 16, compaction copies 64, merge list capture 4
         perParticle += 24ULL + 16ULL + 64ULL + 4ULL;
     }
-    // boundary slot (whole groups of 16): position + psi 16, viscosity 4,
-    // group box 2; cell: count, offset and boundary offset 12, boundary box 32.
-    return perParticle * capacity + 22ULL * boundarySlots(boundaryCount) + 44ULL * (cells + 1ULL);
+    // boundary slot: position + psi 16, viscosity 4, chunk box 32; cell: count,
+    // offset and boundary offset 12, boundary box 32, wall distance bound 4.
+    return perParticle * capacity + 52ULL * boundaryCount + 48ULL * (cells + 1ULL);
 }
 
 // --------------------------------------------------------------- engine --
@@ -218,11 +214,7 @@ T* Engine::alloc(size_t n)
 }
 
 Engine::Engine(const EngineConfig& cfg)
-    : cfg_(cfg), pAlloc_(cfg.particleCapacity),
-      // room for the rounding of every scene's range to whole groups, so that
-      // scenes whose counts sum to boundaryCapacity fit as before
-      bAlloc_(boundarySlots(cfg.boundaryCapacity) + static_cast<uint64_t>(kBoundaryGroup) * cfg.sceneSlots),
-      cAlloc_(cfg.cellCapacity)
+    : cfg_(cfg), pAlloc_(cfg.particleCapacity), bAlloc_(cfg.boundaryCapacity), cAlloc_(cfg.cellCapacity)
 {
     if (cfg_.sceneSlots == 0)
     {
@@ -232,7 +224,7 @@ Engine::Engine(const EngineConfig& cfg)
     try
     {
         const uint32_t P = std::max<uint32_t>(cfg_.particleCapacity, 1);
-        const uint32_t B = static_cast<uint32_t>(std::max<uint64_t>(bAlloc_.capacity(), 1));
+        const uint32_t B = std::max<uint32_t>(cfg_.boundaryCapacity, 1);
         const uint32_t C = std::max<uint32_t>(cfg_.cellCapacity, 2);
         const uint32_t S = cfg_.sceneSlots;
         Ctx& c = ctx_;
@@ -280,8 +272,9 @@ Engine::Engine(const EngineConfig& cfg)
         c.bcellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);
         c.bbox0 = alloc<F4>(static_cast<size_t>(C) + 1);
         c.bbox1 = alloc<F4>(static_cast<size_t>(C) + 1);
-        c.bgbox0 = alloc<F4>(B / kBoundaryGroup + 1);
-        c.bgbox1 = alloc<F4>(B / kBoundaryGroup + 1);
+        c.bnear2 = alloc<float>(static_cast<size_t>(C) + 1);
+        c.bgbox0 = alloc<F4>(B);
+        c.bgbox1 = alloc<F4>(B);
         if (cfg_.adaptive)
         {
             c.ckeysA = alloc<uint64_t>(P);
@@ -478,7 +471,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     {
         throw Error(ASPH_ERR_CAPACITY, "particle pool exhausted");
     }
-    const uint64_t bOff = bAlloc_.alloc(boundarySlots(nb));
+    const uint64_t bOff = bAlloc_.alloc(nb);
     if (bOff == UINT64_MAX)
     {
         pAlloc_.release(pOff, capacity);
@@ -488,7 +481,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     if (cBase == UINT64_MAX)
     {
         pAlloc_.release(pOff, capacity);
-        bAlloc_.release(bOff, boundarySlots(nb));
+        bAlloc_.release(bOff, nb);
         throw Error(ASPH_ERR_CAPACITY, "cell pool exhausted");
     }
     synchronize();
@@ -703,29 +696,110 @@ void Engine::uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32
     }
     hipCheck(hipMemcpy(ctx_.bbox0 + d.cellBase, lo.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
     hipCheck(hipMemcpy(ctx_.bbox1 + d.cellBase, hi.data(), sizeof(F4) * (cells + 1), hipMemcpyHostToDevice), "upload");
-    // group boxes over the sorted boundary slots: a cell at an edge or a
-    // corner of the domain collects several walls, so its box reaches into the
-    // fluid, while a group of consecutive slots is one wall's strip
+    // Chunk boxes: each cell's boundary slots in chunks of kBoundaryGroup from
+    // the cell's first slot, the box of a chunk stored at its first slot. A
+    // cell at an edge or a corner of the domain collects several walls, so its
+    // own box reaches into the fluid, while a chunk (consecutive slots of one
+    // cell, in generation order) is a strip of one wall.
     if (nb > 0)
     {
-        const uint32_t g0 = d.bOff / kBoundaryGroup;
-        const uint32_t ng = (nb + kBoundaryGroup - 1) / kBoundaryGroup;
-        std::vector<F4> glo(ng, F4 {inf, inf, inf, 0.0F});
-        std::vector<F4> ghi(ng, F4 {-inf, -inf, -inf, 0.0F});
+        std::vector<F4> clo(nb, F4 {inf, inf, inf, 0.0F});
+        std::vector<F4> chi(nb, F4 {-inf, -inf, -inf, 0.0F});
+        std::vector<uint32_t> starts;
         for (uint32_t k = 0; k < nb; ++k)
         {
-            const uint32_t g = k / kBoundaryGroup;  // bOff is a multiple of the group size
+            const bool first = k == 0 || hs.boundaryKeysSorted[k] != hs.boundaryKeysSorted[k - 1];
+            if (first || (k - starts.back()) == kBoundaryGroup)
+            {
+                starts.push_back(k);
+            }
+            const uint32_t k0 = starts.back();
             const F4& p = s0[k];
-            glo[g].x = std::min(glo[g].x, p.x);
-            glo[g].y = std::min(glo[g].y, p.y);
-            glo[g].z = std::min(glo[g].z, p.z);
-            ghi[g].x = std::max(ghi[g].x, p.x);
-            ghi[g].y = std::max(ghi[g].y, p.y);
-            ghi[g].z = std::max(ghi[g].z, p.z);
+            clo[k0].x = std::min(clo[k0].x, p.x);
+            clo[k0].y = std::min(clo[k0].y, p.y);
+            clo[k0].z = std::min(clo[k0].z, p.z);
+            chi[k0].x = std::max(chi[k0].x, p.x);
+            chi[k0].y = std::max(chi[k0].y, p.y);
+            chi[k0].z = std::max(chi[k0].z, p.z);
         }
-        hipCheck(hipMemcpy(ctx_.bgbox0 + g0, glo.data(), sizeof(F4) * ng, hipMemcpyHostToDevice), "upload");
-        hipCheck(hipMemcpy(ctx_.bgbox1 + g0, ghi.data(), sizeof(F4) * ng, hipMemcpyHostToDevice), "upload");
+        hipCheck(hipMemcpy(ctx_.bgbox0 + d.bOff, clo.data(), sizeof(F4) * nb, hipMemcpyHostToDevice), "upload");
+        hipCheck(hipMemcpy(ctx_.bgbox1 + d.bOff, chi.data(), sizeof(F4) * nb, hipMemcpyHostToDevice), "upload");
+        std::vector<F4> glo;
+        std::vector<F4> ghi;
+        glo.reserve(starts.size());
+        ghi.reserve(starts.size());
+        for (const uint32_t k0 : starts)
+        {
+            glo.push_back(clo[k0]);
+            ghi.push_back(chi[k0]);
+        }
+        nearBounds(d, glo, ghi);
     }
+    else
+    {
+        std::vector<float> far(cells, std::numeric_limits<float>::max());
+        hipCheck(hipMemcpy(ctx_.bnear2 + d.cellBase, far.data(), sizeof(float) * cells, hipMemcpyHostToDevice),
+                 "upload");
+    }
+}
+
+// Per cell of the scene, a lower bound of the squared distance from any point
+// of the cell (widened by a margin that covers the rounding of cell keys) to
+// any boundary particle, from the boxes of the boundary chunks; capped at two
+// cell widths. A particle whose squared support (2h)^2 lies below its cell's
+// bound has no boundary particle within 2h, so its boundary walks add
+// nothing and are skipped. The bound is rounded down with a relative margin
+// far above the rounding of the walks' own distance tests.
+void Engine::nearBounds(const SceneDev& d, const std::vector<F4>& glo, const std::vector<F4>& ghi)
+{
+    const uint32_t cells = d.nCells;
+    const double cs[3] = {d.cs[0], d.cs[1], d.cs[2]};
+    const double mn[3] = {d.dmin[0], d.dmin[1], d.dmin[2]};
+    const int gdim[3] = {d.gx, d.gy, d.gz};
+    const double cap = 2.0 * std::max(cs[0], std::max(cs[1], cs[2]));
+    std::vector<double> near(cells, cap * cap);
+    for (size_t g = 0; g < glo.size(); ++g)
+    {
+        const double lo[3] = {glo[g].x, glo[g].y, glo[g].z};
+        const double hi[3] = {ghi[g].x, ghi[g].y, ghi[g].z};
+        int i0[3];
+        int i1[3];
+        for (int a = 0; a < 3; ++a)
+        {
+            i0[a] = std::max(static_cast<int>(std::floor((lo[a] - cap - mn[a]) / cs[a])) - 1, 0);
+            i1[a] = std::min(static_cast<int>(std::floor((hi[a] + cap - mn[a]) / cs[a])) + 1, gdim[a] - 1);
+        }
+        for (int z = i0[2]; z <= i1[2]; ++z)
+        {
+            for (int y = i0[1]; y <= i1[1]; ++y)
+            {
+                for (int x = i0[0]; x <= i1[0]; ++x)
+                {
+                    const int idx[3] = {x, y, z};
+                    double d2 = 0.0;
+                    for (int a = 0; a < 3; ++a)
+                    {
+                        const double m = 1e-3 * cs[a];
+                        const double cl = mn[a] + idx[a] * cs[a] - m;
+                        const double ch = mn[a] + (idx[a] + 1) * cs[a] + m;
+                        const double gap = std::max(0.0, std::max(lo[a] - ch, cl - hi[a]));
+                        d2 += gap * gap;
+                    }
+                    const size_t cell =
+                        static_cast<size_t>(x) + static_cast<size_t>(y) * gdim[0] +
+                        static_cast<size_t>(z) * static_cast<size_t>(gdim[0]) * static_cast<size_t>(gdim[1]);
+                    near[cell] = std::min(near[cell], d2);
+                }
+            }
+        }
+    }
+    std::vector<float> out(cells);
+    for (uint32_t k = 0; k < cells; ++k)
+    {
+        const float f = static_cast<float>(near[k] * (1.0 - 1e-5));
+        out[k] = f > 0.0F ? std::nextafter(f, 0.0F) : 0.0F;
+    }
+    hipCheck(hipMemcpy(ctx_.bnear2 + d.cellBase, out.data(), sizeof(float) * cells, hipMemcpyHostToDevice), "upload");
 }
 
 void Engine::uploadScene(uint32_t id)
@@ -792,7 +866,7 @@ void Engine::removeScene(uint32_t id)
     synchronize();
     HostScene& hs = scenes_[id];
     pAlloc_.release(hs.dev.pOff, hs.dev.pCap);
-    bAlloc_.release(hs.dev.bOff, boundarySlots(hs.dev.bCount));
+    bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
     cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
     hs.active = false;
     hs.dev.flags = 0;
@@ -838,7 +912,7 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     const GridGeom g = makeGrid(dom, hs.dev.h0);
     const uint64_t cells = g.cells();
     cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
-    bAlloc_.release(hs.dev.bOff, boundarySlots(hs.dev.bCount));
+    bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
     // NeighborGrid::updateBoundarySize reallocates the cell arrays for the new
     // domain (NeighborGrid.cu:51-85); the pools grow when the new grid or wall
     // lattice does not fit.
@@ -848,11 +922,11 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
         growCells(cAlloc_.capacity() + cells + 1);
         cBase = cAlloc_.alloc(cells + 1);
     }
-    uint64_t bOff = bAlloc_.alloc(boundarySlots(nb));
+    uint64_t bOff = bAlloc_.alloc(nb);
     if (bOff == UINT64_MAX)
     {
-        growBoundary(bAlloc_.capacity() + boundarySlots(nb));
-        bOff = bAlloc_.alloc(boundarySlots(nb));
+        growBoundary(bAlloc_.capacity() + nb);
+        bOff = bAlloc_.alloc(nb);
     }
     if (cBase == UINT64_MAX || bOff == UINT64_MAX)
     {
@@ -916,6 +990,7 @@ void Engine::growCells(uint64_t newCap)
     regrow(ctx_.bcellOff, oldC + 1, C + 1);
     regrow(ctx_.bbox0, oldC + 1, C + 1);
     regrow(ctx_.bbox1, oldC + 1, C + 1);
+    regrow(ctx_.bnear2, oldC + 1, C + 1);
     regrow(tileSums_, scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(oldC) + 1)),
            scanTileWords(std::max<uint32_t>(ctx_.pCap, static_cast<uint32_t>(C) + 1)));
     ctx_.cCap = static_cast<uint32_t>(C);
@@ -931,8 +1006,8 @@ void Engine::growBoundary(uint64_t newCap)
     const size_t B = std::max<size_t>(newCap, oldB);
     regrow(ctx_.bs0, oldB, B);
     regrow(ctx_.bvisc, oldB, B);
-    regrow(ctx_.bgbox0, oldB / kBoundaryGroup + 1, B / kBoundaryGroup + 1);
-    regrow(ctx_.bgbox1, oldB / kBoundaryGroup + 1, B / kBoundaryGroup + 1);
+    regrow(ctx_.bgbox0, oldB, B);
+    regrow(ctx_.bgbox1, oldB, B);
     bAlloc_.grow(B);
     invalidateGraphs();
 }

@@ -156,6 +156,7 @@ private:
     void uploadOrder();
     void updateExtents();
     void uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32_t nb);
+    void nearBounds(const SceneDev& d, const std::vector<F4>& glo, const std::vector<F4>& ghi);
     void enqueueStep(bool refine, std::vector<hipEvent_t>* marks);
     bool anyRefineNext() const;
     void advanceFrames();

@@ -92,8 +92,11 @@ struct Ctx
     // (min xyz, max xyz; empty cells hold an inverted box)
     F4* bbox0;
     F4* bbox1;
-    F4* bgbox0;  // per kBoundaryGroup boundary slots: box of their positions
-    F4* bgbox1;
+    F4* bgbox0;  // at the first slot of each chunk of kBoundaryGroup slots of a
+    F4* bgbox1;  // cell's boundary range: the box of the chunk's positions
+    // per cell: a lower bound of the squared distance from any point of the
+    // cell to any boundary particle (capped at two cell widths)
+    float* bnear2;
 
     // refinement scratch (dense-indexed)
     uint64_t* ckeysA;

@@ -396,6 +396,13 @@ __device__ __forceinline__ void walkFluid(const Ctx& c, const SceneDev& sc, uint
     }
 }
 
+// False if no boundary particle can lie within R of sorted slot t (the
+// per-cell bound of the squared distance to the walls exceeds R^2).
+__device__ __forceinline__ bool wallsMayReach(const Ctx& c, uint32_t t, float R)
+{
+    return R * R >= c.bnear2[c.sortedKeys[t]];
+}
+
 __device__ __forceinline__ bool touchesEdge(const SceneDev& sc, const CellBox& b)
 {
     if ((sc.flags & kSceneBoundaryInside) != 0U)
@@ -744,20 +751,23 @@ __device__ __forceinline__ float boundaryDensity(const Ctx& c, const SceneDev& s
             for (int z = b.z0; z <= b.z1; ++z)
             {
                 const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
+                const uint32_t start = c.bcellOff[cell];
+                const uint32_t lo = max(start, sc.bLostEnd);
                 const uint32_t hiS = c.bcellOff[cell + 1U];
                 if (lo >= hiS || !boxMayReach(c.bbox0[cell], c.bbox1[cell], q, r2))
                 {
                     continue;
                 }
-                for (uint32_t g = lo / kBoundaryGroup; g <= (hiS - 1U) / kBoundaryGroup; ++g)
+                // chunks of the cell's range, each culled by its box
+                for (uint32_t k0 = start + (lo - start) / kBoundaryGroup * kBoundaryGroup; k0 < hiS;
+                     k0 += kBoundaryGroup)
                 {
-                    if (!boxMayReach(c.bgbox0[g], c.bgbox1[g], q, r2))
+                    if (!boxMayReach(c.bgbox0[k0], c.bgbox1[k0], q, r2))
                     {
                         continue;
                     }
-                    const uint32_t jhi = min(hiS, (g + 1U) * kBoundaryGroup);
-                    for (uint32_t j = max(lo, g * kBoundaryGroup); j < jhi; ++j)
+                    const uint32_t jhi = min(hiS, k0 + kBoundaryGroup);
+                    for (uint32_t j = max(lo, k0); j < jhi; ++j)
                     {
                         const F4 o = c.bs0[j];
                         const float dx = o.x - q.x;
@@ -783,7 +793,7 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
                                               float hi, uint32_t t, float rho)
 {
     float rhoB = 0.0F;
-    if (sc.bCount > 0U && touchesEdge(sc, b))
+    if (sc.bCount > 0U && touchesEdge(sc, b) && wallsMayReach(c, t, 2.0F * hi))
     {
         rhoB = boundaryDensity(c, sc, b, q, hi);
     }
@@ -1031,20 +1041,23 @@ __device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDe
             for (int z = b.z0; z <= b.z1; ++z)
             {
                 const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
-                const uint32_t lo = max(c.bcellOff[cell], sc.bLostEnd);
+                const uint32_t start = c.bcellOff[cell];
+                const uint32_t lo = max(start, sc.bLostEnd);
                 const uint32_t hiS = c.bcellOff[cell + 1U];
                 if (lo >= hiS || !boxMayReach(c.bbox0[cell], c.bbox1[cell], q, r2))
                 {
                     continue;
                 }
-                for (uint32_t g = lo / kBoundaryGroup; g <= (hiS - 1U) / kBoundaryGroup; ++g)
+                // chunks of the cell's range, each culled by its box
+                for (uint32_t k0 = start + (lo - start) / kBoundaryGroup * kBoundaryGroup; k0 < hiS;
+                     k0 += kBoundaryGroup)
                 {
-                    if (!boxMayReach(c.bgbox0[g], c.bgbox1[g], q, r2))
+                    if (!boxMayReach(c.bgbox0[k0], c.bgbox1[k0], q, r2))
                     {
                         continue;
                     }
-                    const uint32_t jhi = min(hiS, (g + 1U) * kBoundaryGroup);
-                    for (uint32_t j = max(lo, g * kBoundaryGroup); j < jhi; ++j)
+                    const uint32_t jhi = min(hiS, k0 + kBoundaryGroup);
+                    for (uint32_t j = max(lo, k0); j < jhi; ++j)
                     {
                         const F4 o = c.bs0[j];
                         const float ox = q.x - o.x;
@@ -1119,7 +1132,7 @@ __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, co
         az = az + k * A.asz;
     }
     BoundaryAcc B {0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
-    if (sc.bCount > 0U && touchesEdge(sc, b))
+    if (sc.bCount > 0U && touchesEdge(sc, b) && wallsMayReach(c, t, 2.0F * ri.z))
     {
         B = boundaryForce(c, sc, b, q, vi, ri, epsH2);
     }

@@ -77,7 +77,7 @@ def pools_for_budget(budget_bytes: int, list_capacity: int = 128, adaptive: bool
     fits `budget_bytes`, with boundary and cell pools sized per fluid slot."""
     fp, fb, fc, _ = unit_bytes(list_capacity, adaptive)
     per = fp + boundary_ratio * fb + cell_ratio * fc
-    p = int(min(budget_bytes // per, 2**32 - 2))
+    p = int(min((budget_bytes - 2 * fc) // per, 2**32 - 2))  # 2 cells beyond the ratio
     return p, int(p * boundary_ratio), int(p * cell_ratio) + 2
 
 
