@@ -235,6 +235,8 @@ Engine::Engine(const EngineConfig& cfg)
         c.sortedStarts = alloc<uint32_t>(S);
         cellOrderDev_ = alloc<uint32_t>(S);
         c.cellOrder = cellOrderDev_;
+        cellBaseOrderDev_ = alloc<uint32_t>(S);
+        c.cellBaseOrder = cellBaseOrderDev_;
         nOrderDev_ = alloc<uint32_t>(1);
         c.nOrderPtr = nOrderDev_;
         c.nTotal = alloc<uint32_t>(1);
@@ -250,7 +252,6 @@ Engine::Engine(const EngineConfig& cfg)
         c.valsA = alloc<uint32_t>(P);
         c.keysB = alloc<uint32_t>(P);
         c.valsB = alloc<uint32_t>(P);
-        c.cellCount = alloc<uint32_t>(C);
         c.cellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);
         c.cCap = C;
         c.s0 = alloc<F4>(P);
@@ -304,7 +305,6 @@ Engine::Engine(const EngineConfig& cfg)
 
         hipCheck(hipMemset(c.states, 0, sizeof(SceneState) * S), "hipMemset");
         hipCheck(hipMemset(const_cast<SceneDev*>(c.scenes), 0, sizeof(SceneDev) * S), "hipMemset");
-        hipCheck(hipMemset(c.cellCount, 0, sizeof(uint32_t) * C), "hipMemset");
         hipCheck(hipMemset(c.cellOff, 0, sizeof(uint32_t) * (C + 1)), "hipMemset");
         hipCheck(hipMemset(c.bcellOff, 0, sizeof(uint32_t) * (C + 1)), "hipMemset");
         hipCheck(hipMemset(nOrderDev_, 0, sizeof(uint32_t)), "hipMemset");
@@ -590,7 +590,6 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
         hipCheck(hipMemcpy(ctx_.h + o, hh.data(), sizeof(float) * capacity, hipMemcpyHostToDevice), "upload");
         hipCheck(hipMemcpy(ctx_.m + o, mm.data(), sizeof(float) * capacity, hipMemcpyHostToDevice), "upload");
     }
-    hipCheck(hipMemset(ctx_.cellCount + cBase, 0, sizeof(uint32_t) * (cells + 1)), "hipMemset");
     hipCheck(hipMemset(ctx_.cellOff + cBase, 0, sizeof(uint32_t) * (cells + 1)), "hipMemset");
     uploadBoundary(id, boundary, nb);
     SceneState st {};
@@ -825,7 +824,13 @@ void Engine::uploadOrder()
     const uint32_t n = static_cast<uint32_t>(order.size());
     if (n > 0)
     {
+        std::vector<uint32_t> bases(n);
+        for (uint32_t k = 0; k < n; ++k)
+        {
+            bases[k] = scenes_[order[k]].dev.cellBase;
+        }
         hipCheck(hipMemcpy(cellOrderDev_, order.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice), "upload");
+        hipCheck(hipMemcpy(cellBaseOrderDev_, bases.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice), "upload");
     }
     hipCheck(hipMemcpy(nOrderDev_, &n, sizeof(uint32_t), hipMemcpyHostToDevice), "upload");
     updateExtents();
@@ -950,7 +955,6 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     d.nCells = static_cast<uint32_t>(cells);
     d.bOff = static_cast<uint32_t>(bOff);
     hs.cellRegion = static_cast<uint32_t>(cells + 1);
-    hipCheck(hipMemset(ctx_.cellCount + cBase, 0, sizeof(uint32_t) * (cells + 1)), "hipMemset");
     hipCheck(hipMemset(ctx_.cellOff + cBase, 0, sizeof(uint32_t) * (cells + 1)), "hipMemset");
     uploadBoundary(id, boundary, nb);
     uploadScene(id);
@@ -985,7 +989,6 @@ void Engine::growCells(uint64_t newCap)
     synchronize();
     const size_t oldC = ctx_.cCap;
     const size_t C = std::max<size_t>(newCap, oldC);
-    regrow(ctx_.cellCount, oldC, C);
     regrow(ctx_.cellOff, oldC + 1, C + 1);
     regrow(ctx_.bcellOff, oldC + 1, C + 1);
     regrow(ctx_.bbox0, oldC + 1, C + 1);
@@ -1067,9 +1070,10 @@ void Engine::advanceFrames()
     }
 }
 
-const char* const kStageNames[] = {"prep",           "kick_drift_key", "cell_sort",  "cell_scan", "scene_ranges",
-                                   "gather",         "neighbors",      "density",    "force",     "merge_criteria",
-                                   "criterion_sort", "merge",          "compaction", "split",     "kick_refined"};
+const char* const kStageNames[] = {"prep",           "kick_drift_key", "cell_sort",    "gather",
+                                   "neighbors",      "density",        "force",        "merge_criteria",
+                                   "criterion_sort", "merge",          "compaction",   "split",
+                                   "kick_refined"};
 constexpr uint32_t kStageCount = sizeof(kStageNames) / sizeof(kStageNames[0]);
 
 void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
@@ -1093,10 +1097,6 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     mark();
     radixSortU32(c.keysA, c.valsA, c.keysB, c.valsB, c.nTotal, c.nMax, keyBits_, lc.radixBits, hist_, digitTotals_,
                  s);
-    mark();
-    exclusiveScanU32(c.cellCount, c.cellOff, nullptr, c.cExtent, tileSums_, scanTotal_, true, s);
-    mark();
-    launchSceneRanges(c, s);
     mark();
     launchGather(c, lc, s);
     mark();

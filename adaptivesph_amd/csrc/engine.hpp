@@ -185,6 +185,7 @@ private:
     uint32_t* scanTotal_ = nullptr;
     uint32_t* nOrderDev_ = nullptr;
     uint32_t* cellOrderDev_ = nullptr;
+    uint32_t* cellBaseOrderDev_ = nullptr;
     double* infoD_ = nullptr;
     uint32_t* infoU_ = nullptr;
     uint32_t* diagTotal_ = nullptr;

@@ -37,6 +37,7 @@ struct Ctx
     uint32_t* denseStarts;      // [sceneSlots] exclusive prefix of counts
     uint32_t* sortedStarts;     // [sceneSlots] first sorted slot, in cell-base order
     const uint32_t* cellOrder;  // [sceneSlots] scene ids in cell-base order
+    const uint32_t* cellBaseOrder;  // [sceneSlots] their first cells
     const uint32_t* nOrderPtr;  // device: active scenes in cellOrder
     uint32_t* nTotal;           // device: active particles this step
 
@@ -62,7 +63,6 @@ struct Ctx
     uint32_t* valsB;
     uint32_t* sortedKeys;  // result buffers of the sort (set per step)
     uint32_t* sortedVals;
-    uint32_t* cellCount;  // [cCap]
     uint32_t* cellOff;    // [cCap + 1]
     uint32_t cCap;
 
@@ -129,7 +129,6 @@ struct LaunchCfg
 // Step stages, in call order.
 void launchPrep(const Ctx& c, hipStream_t s);
 void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
-void launchSceneRanges(const Ctx& c, hipStream_t s);
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s);

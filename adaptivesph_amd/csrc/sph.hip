@@ -193,23 +193,6 @@ __global__ __launch_bounds__(1024) void kKickDriftKey(Ctx c)
         const uint32_t key = sc.cellBase + cellKey(sc, p.x, p.y, p.z);
         c.keysA[d] = key;
         c.valsA[d] = i;
-        // cell histogram: one atomic per distinct key of the wave (neighbouring
-        // particles of the canonical order mostly share a cell)
-        uint64_t todo = __ballot(1);
-        while (todo != 0ULL)
-        {
-            const uint32_t k0 = __builtin_amdgcn_readfirstlane(key);
-            const uint64_t same = __ballot(key == k0 ? 1 : 0);
-            if (key == k0)
-            {
-                if (__lane_id() == static_cast<uint32_t>(__ffsll(static_cast<long long>(same)) - 1))
-                {
-                    atomicAdd(&c.cellCount[k0], static_cast<uint32_t>(__popcll(same)));
-                }
-                break;
-            }
-            todo &= ~same;
-        }
     }
 }
 
@@ -218,22 +201,45 @@ __global__ __launch_bounds__(1024) void kKickDriftKey(Ctx c)
 // calculateCellStartAndEndIndices (NeighborKernels.cu:27-36): when the
 // lowest-key cell of a scene holds exactly one particle, that cell and the
 // next occupied cell are invisible to every neighbour walk.
-__global__ void kSceneRanges(Ctx c)
+// First sorted slot whose key is at least `key` (keys ascending over [0, n)).
+__device__ __forceinline__ uint32_t lowerBound(const uint32_t* keys, uint32_t n, uint32_t key)
 {
-    const uint32_t nOrder = *c.nOrderPtr;
+    uint32_t lo = 0;
+    uint32_t hi = n;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1U;
+        if (keys[mid] < key)
+        {
+            lo = mid + 1U;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+// Per scene in cell-base order: its first sorted slot, and the end of the
+// slots that the reference's cell start/end construction hides from every
+// walk (NeighborKernels.cu:27-36: with the slot-0 omission, the lowest-key
+// cell is lost if it holds one particle, together with the next cell).
+__device__ void sceneRanges(const Ctx& c, uint32_t n, uint32_t nOrder)
+{
     for (uint32_t k = threadIdx.x; k < nOrder; k += blockDim.x)
     {
         const uint32_t s = c.cellOrder[k];
         const SceneDev& sc = c.scenes[s];
         SceneState& st = c.states[s];
-        const uint32_t ss = c.cellOff[sc.cellBase];
+        const uint32_t ss = lowerBound(c.sortedKeys, n, sc.cellBase);
         c.sortedStarts[k] = ss;
         st.sortedStart = ss;
-        const uint32_t n = st.count;
+        const uint32_t cnt = st.count;
         uint32_t lost = ss;
-        if ((sc.flags & kSceneFixCellStart) == 0U && n > 0U)
+        if ((sc.flags & kSceneFixCellStart) == 0U && cnt > 0U)
         {
-            if (n == 1U)
+            if (cnt == 1U)
             {
                 lost = ss + 1U;
             }
@@ -243,7 +249,7 @@ __global__ void kSceneRanges(Ctx c)
                 const uint32_t k1 = c.sortedKeys[ss + 1U];
                 if (k0 != k1)
                 {
-                    lost = c.cellOff[k1 + 1U];
+                    lost = lowerBound(c.sortedKeys, n, k1 + 1U);
                 }
             }
         }
@@ -251,16 +257,23 @@ __global__ void kSceneRanges(Ctx c)
     }
 }
 
-// --------------------------------------------------------------- gather --
-// Sorted shadow copies of the particle state and, per aligned group of
-// kGroup sorted slots, the bounding box of the group with the largest
-// (2h)^2 of its members (used to skip groups no member of which can pass the
-// neighbour test of a query).
+// Sorted shadows and group boxes; also the cell offsets of the sorted order
+// (CSR: cellOff[c] = first slot whose cell is at least c, the exclusive scan
+// of the cell histogram) and, in block 0, the per-scene ranges. The kernels
+// after this one read both.
 __global__ __launch_bounds__(1024) void kGather(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t nOrder = *c.nOrderPtr;
     const uint32_t nPad = (n + 63U) & ~63U;
+    if (blockIdx.x == 0U)
+    {
+        sceneRanges(c, n, nOrder);
+    }
+    for (uint32_t cell = blockIdx.x * blockDim.x + threadIdx.x; cell < c.cExtent; cell += gridDim.x * blockDim.x)
+    {
+        c.cellOff[cell] = lowerBound(c.sortedKeys, n, cell);
+    }
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nPad; t += gridDim.x * blockDim.x)
     {
         float lx = FLT_MAX, ly = FLT_MAX, lz = FLT_MAX;
@@ -268,7 +281,7 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
         float mr = 0.0F;
         if (t < n)
         {
-            const uint32_t k = segmentOf(c.sortedStarts, nOrder, t);
+            const uint32_t k = segmentOf(c.cellBaseOrder, nOrder, c.sortedKeys[t]);
             const uint32_t i = c.sortedVals[t];
             const F4 p = c.pos[i];
             const F4 v = c.vel[i];
@@ -2109,7 +2122,6 @@ void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     hipLaunchKernelGGL(kKickDriftKey, dim3(gridFor(c.nMax, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
 }
 
-void launchSceneRanges(const Ctx& c, hipStream_t s) { hipLaunchKernelGGL(kSceneRanges, dim3(1), dim3(256), 0, s, c); }
 
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {

@@ -52,6 +52,7 @@ def load() -> C.CDLL:
         "oracle_grid_dims": (None, [vp, P(C.c_int32)]),
         "oracle_rebuild_grid": (None, [vp]),
         "oracle_grid": (None, [vp, vp, vp, vp, vp]),
+        "oracle_grid_count": (C.c_uint32, [vp]),
         "oracle_boundary_grid": (None, [vp, vp, vp, vp, vp]),
         "oracle_neighbor_sets": (None, [vp, vp, vp]),
         "oracle_average_neighbor_count": (f32, [vp]),
@@ -188,7 +189,8 @@ class OracleSim:
 
     def grid(self, boundary: bool = False) -> dict[str, np.ndarray]:
         gx, gy, gz = self.grid_dims()
-        n = self.count() if not boundary else None
+        # the grid of the last update (its particle count, not the current one)
+        n = int(self._lib.oracle_grid_count(self._h)) if not boundary else None
         if boundary:
             raise NotImplementedError
         out = {

@@ -1692,6 +1692,15 @@ void oracle_grid_dims(oracle_sim* s, int32_t* dims)
 // update is what oracle_grid returns otherwise).
 void oracle_rebuild_grid(oracle_sim* s) { ORACLE_DISPATCH(s, S.buildGrid(S.fluid, S.pos, S.n)); }
 
+// Number of particles in the fluid grid (the count when it was built; a
+// refinement step changes the count afterwards).
+uint32_t oracle_grid_count(oracle_sim* s)
+{
+    uint32_t n = 0;
+    ORACLE_DISPATCH(s, n = static_cast<uint32_t>(S.fluid.keys.size()));
+    return n;
+}
+
 void oracle_grid(oracle_sim* s, int32_t* keys, int32_t* idx, int32_t* cellStart, int32_t* cellEnd)
 {
     ORACLE_DISPATCH(s, {
