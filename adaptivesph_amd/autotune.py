@@ -4,7 +4,8 @@ Three launch parameters shape a step, each acting on its own stages:
 
 * block_stream   threads per block of the streaming kernels (kick/drift/key,
                  gather);
-* radix_bits     digit width of the cell-key radix sort (cell sort);
+* radix_bits     digit width of the cell-key radix sort (cell sort; 0 = the
+                 width with the fewest passes);
 * block_neighbor threads per block of the neighbour kernels (density, forces).
 
 The tuner measures, in a fixed order, each candidate value on the stages it
@@ -23,10 +24,10 @@ import numpy as np
 
 SPACE = {
     "block_stream": ((128, 256, 512, 1024), ("kick_drift_key", "gather")),
-    "radix_bits": ((4, 6, 7, 8), ("cell_sort",)),
+    "radix_bits": ((0, 4, 6, 7, 8, 10), ("cell_sort",)),
     "block_neighbor": ((64, 128, 256), ("neighbors", "density", "force")),
 }
-DEFAULT = {"block_stream": 256, "radix_bits": 8, "block_neighbor": 256}
+DEFAULT = {"block_stream": 256, "radix_bits": 0, "block_neighbor": 256}
 
 
 def _apply(batch, shape: dict, graph: bool) -> None:

@@ -113,7 +113,7 @@ class Batch:
                     "copy scene")
         return out
 
-    def set_launch(self, block_stream: int = 256, radix_bits: int = 8, graph: bool = True,
+    def set_launch(self, block_stream: int = 256, radix_bits: int = 0, graph: bool = True,
                    block_neighbor: int = 256) -> None:
         self._check(self._lib.asph_batch_set_launch(self._h, block_stream, block_neighbor, radix_bits, int(graph)),
                     "set launch")

@@ -434,9 +434,10 @@ int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t blo
         {
             throw asph::Error(ASPH_ERR_INVALID, "block_neighbor must be 64, 128 or 256");
         }
-        if (radix_bits != 4 && radix_bits != 6 && radix_bits != 7 && radix_bits != 8)
+        if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 7 && radix_bits != 8 &&
+            radix_bits != 10)
         {
-            throw asph::Error(ASPH_ERR_INVALID, "radix_bits must be 4, 6, 7 or 8");
+            throw asph::Error(ASPH_ERR_INVALID, "radix_bits must be 0 (from the key width), 4, 6, 7, 8 or 10");
         }
         asph::LaunchCfg lc = e.config().launch;
         lc.blockStream = block_stream;

@@ -295,8 +295,8 @@ Engine::Engine(const EngineConfig& cfg)
             c.xm = alloc<float>(P);
             c.mergeListDbg = alloc<uint32_t>(P);
         }
-        hist_ = alloc<uint32_t>(radixHistWords(P, 8));
-        digitTotals_ = alloc<uint32_t>(256);
+        hist_ = alloc<uint32_t>(radixHistWords(P, kMaxRadixBits));
+        digitTotals_ = alloc<uint32_t>(1U << kMaxRadixBits);
         tileSums_ = alloc<uint32_t>(scanTileWords(std::max<uint32_t>(P, C + 1)));
         scanTotal_ = alloc<uint32_t>(1);
         infoD_ = alloc<double>(kInfoBlocks);
@@ -373,7 +373,8 @@ void Engine::invalidateGraphs()
 void Engine::buildCtx()
 {
     Ctx& c = ctx_;
-    const int passes = (keyBits_ + cfg_.launch.radixBits - 1) / cfg_.launch.radixBits;
+    const int rb = radixDigitBits(cfg_.launch.radixBits, keyBits_);
+    const int passes = (keyBits_ + rb - 1) / rb;
     sortParity_ = passes & 1;
     c.sortedKeys = sortParity_ != 0 ? c.keysB : c.keysA;
     c.sortedVals = sortParity_ != 0 ? c.valsB : c.valsA;
@@ -1095,7 +1096,8 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     mark();
     launchKickDriftKey(c, lc, s);
     mark();
-    radixSortU32(c.keysA, c.valsA, c.keysB, c.valsB, c.nTotal, c.nMax, keyBits_, lc.radixBits, hist_, digitTotals_,
+    radixSortU32(c.keysA, c.valsA, c.keysB, c.valsB, c.nTotal, c.nMax, keyBits_,
+                 radixDigitBits(lc.radixBits, keyBits_), hist_, digitTotals_,
                  s);
     mark();
     launchGather(c, lc, s);

@@ -15,6 +15,10 @@ namespace asph
 
 // ------------------------------------------------------------ sort.hip --
 size_t radixHistWords(uint32_t nMax, int radixBits);
+// Digit width of the cell-key sort: `requested` (4, 6, 7, 8 or 10), or for 0
+// the width with the fewest passes over keyBits bits.
+int radixDigitBits(int requested, int keyBits);
+constexpr int kMaxRadixBits = 10;
 size_t scanTileWords(uint32_t nMax);
 // Returns 0 when the result is in (keysA, valsA), 1 when in (keysB, valsB).
 int radixSortU32(uint32_t* keysA, uint32_t* valsA, uint32_t* keysB, uint32_t* valsB, const uint32_t* nPtr,
@@ -123,7 +127,7 @@ struct LaunchCfg
 {
     uint32_t blockStream = 256;  // streaming kernels
     uint32_t blockNeighbor = 256;  // neighbour kernels (64, 128 or 256)
-    int radixBits = 8;
+    int radixBits = 0;  // 0: chosen from the key width (radixDigitBits)
 };
 
 // Step stages, in call order.

@@ -401,6 +401,18 @@ size_t radixHistWords(uint32_t nMax, int radixBits)
 
 size_t scanTileWords(uint32_t nMax) { return (nMax + kTile - 1) / kTile + 1; }
 
+int radixDigitBits(int requested, int keyBits)
+{
+    if (requested != 0)
+    {
+        return requested;
+    }
+    // the fewest passes; 8-bit digits when 10-bit ones would not save one
+    const int passes8 = (keyBits + 7) / 8;
+    const int passes10 = (keyBits + 9) / 10;
+    return passes10 < passes8 ? 10 : 8;
+}
+
 int radixSortU32(uint32_t* keysA, uint32_t* valsA, uint32_t* keysB, uint32_t* valsB, const uint32_t* nPtr,
                  uint32_t nMax, int bits, int radixBits, uint32_t* hist, uint32_t* digitTotals, hipStream_t stream)
 {
@@ -412,6 +424,8 @@ int radixSortU32(uint32_t* keysA, uint32_t* valsA, uint32_t* keysB, uint32_t* va
         return radixSortImpl<6, uint32_t>(keysA, valsA, keysB, valsB, nPtr, nMax, bits, hist, digitTotals, stream);
     case 7:
         return radixSortImpl<7, uint32_t>(keysA, valsA, keysB, valsB, nPtr, nMax, bits, hist, digitTotals, stream);
+    case 10:
+        return radixSortImpl<10, uint32_t>(keysA, valsA, keysB, valsB, nPtr, nMax, bits, hist, digitTotals, stream);
     default:
         return radixSortImpl<8, uint32_t>(keysA, valsA, keysB, valsB, nPtr, nMax, bits, hist, digitTotals, stream);
     }

@@ -55,7 +55,8 @@ int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, a
 
 /* Launch shape: threads per block of the streaming kernels (64..1024, a
    multiple of 64) and of the neighbour kernels (64, 128 or 256), radix digit
-   width of the cell-key sort (4, 6, 7 or 8 bits) and graph replay on/off.
+   width of the cell-key sort (4, 6, 7, 8 or 10 bits; 0 picks the width with
+   the fewest passes over the active key width) and graph replay on/off.
    Results do not depend on the launch shape. */
 int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t block_neighbor, uint32_t radix_bits,
                           uint32_t use_graph);

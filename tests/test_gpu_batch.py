@@ -82,7 +82,8 @@ def test_continuous_admission_and_removal():
 def test_launch_shapes_do_not_change_results():
     scenes = _mix()[:3]
     ref = None
-    for block, radix, nb in ((256, 8, 256), (128, 6, 64), (512, 4, 128), (64, 7, 256), (1024, 8, 64)):
+    for block, radix, nb in ((256, 8, 256), (128, 6, 64), (512, 4, 128), (64, 7, 256), (1024, 10, 64),
+                             (256, 0, 128)):
         b = Batch.for_scenes(scenes, graph=True)
         ids = [b.add(s) for s in scenes]
         b.set_launch(block, radix, True, block_neighbor=nb)
