@@ -471,17 +471,17 @@ __device__ __forceinline__ uint32_t xcdBlock()
 
 // Neighbour-list entries whose data are loaded together before the pair math
 // (memory-level parallelism against register footprint).
-#ifndef ASPH_DENSITY_BATCH
-#define ASPH_DENSITY_BATCH 8
-#endif
 #ifndef ASPH_FORCE_BATCH
 #define ASPH_FORCE_BATCH 4
 #endif
-constexpr uint32_t kDensityBatch = ASPH_DENSITY_BATCH;  // a multiple of 4
 #ifndef ASPH_DENSITY_UNIFORM_BATCH
-#define ASPH_DENSITY_UNIFORM_BATCH 12
+#define ASPH_DENSITY_UNIFORM_BATCH 8
 #endif
-constexpr uint32_t kDensityUniformBatch = ASPH_DENSITY_UNIFORM_BATCH;  // a multiple of 4
+constexpr uint32_t kListDensityUniformBatch = ASPH_DENSITY_UNIFORM_BATCH;  // a multiple of 4
+#ifndef ASPH_LIST_DENSITY_BATCH
+#define ASPH_LIST_DENSITY_BATCH 4
+#endif
+constexpr uint32_t kListDensityBatch = ASPH_LIST_DENSITY_BATCH;  // a multiple of 4
 constexpr uint32_t kForceBatch = ASPH_FORCE_BATCH;      // a multiple of 4
 #ifndef ASPH_FORCE_UNIFORM_BATCH
 #define ASPH_FORCE_UNIFORM_BATCH 8
@@ -647,93 +647,6 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
     }
 }
 
-// List builder. BS (the block size) is a compile-time constant so that the
-// staging addresses fold into immediates.
-template <uint32_t BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
-{
-    // staging ring of kRing entries per thread, entry-major ([e][thread]) so
-    // that the lanes' writes fall in distinct banks; per wave, the survivor
-    // table of the candidate filter
-    constexpr uint32_t kRing = 2U * kChunk;
-    constexpr uint32_t kTable = 64U + kStashPad;
-    __shared__ uint32_t stage[kRing * BS];
-    __shared__ F4 stash[(BS / 64U) * kTable];
-    __shared__ uint32_t slots[(BS / 64U) * kTable];
-    const uint32_t n = *c.nTotal;
-    const uint32_t nPad = (n + 63U) & ~63U;
-    const uint32_t K = c.listCap;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63U;
-    F4* wstash = stash + (tid / 64U) * kTable;
-    uint32_t* wslot = slots + (tid / 64U) * kTable;
-    for (uint32_t tw = blockIdx.x * BS + tid; tw < nPad; tw += gridDim.x * BS)
-    {
-        const bool valid = tw < n;
-        const uint32_t t = valid ? tw : n - 1U;
-        const uint32_t s = c.sscene[t];
-        const uint32_t sw = __builtin_amdgcn_readfirstlane(s);
-        const bool oneScene = __ballot((valid && s != sw) ? 1 : 0) == 0ULL;
-        const SceneDev& sc = c.scenes[s];
-        const uint32_t lost = c.states[s].lostEnd;
-        const F4 q = c.s0[t];
-        const float hi = c.shm[t].x;
-        const float R = 2.0F * hi;
-        const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        // entries [done, cnt) are staged at ring positions (done .. cnt-1) mod
-        // kRing; at most kRing - 1 are staged, so position cnt mod kRing is free
-        uint32_t cnt = 0;
-        uint32_t done = 0;
-        // the thread's next chunk: chunk k of slot t is the 16 bytes at
-        // list + (k * pCap + t) * 4 entries
-        uint4* cptr = reinterpret_cast<uint4*>(c.list) + t;
-        const uint32_t cstep = c.pCap;
-        auto push = [&](bool hit, uint32_t j) {
-            stage[(cnt & (kRing - 1U)) * BS + tid] = j;
-            cnt += hit ? 1U : 0U;
-        };
-        // writes the oldest staged chunk once it is full (called after at most
-        // kChunk pushes, so the ring never holds more than kRing - 1 entries)
-        auto drain = [&]() {
-            if (cnt - done >= kChunk)
-            {
-                if (done < K)
-                {
-                    const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                    storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
-                    cptr += cstep;
-                }
-                done += kChunk;
-            }
-        };
-        if (oneScene)
-        {
-            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain);
-        }
-        else if (valid)
-        {
-            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
-                push(true, j);
-                drain();
-            });
-        }
-        if (valid)
-        {
-            if (cnt <= K && cnt > done)
-            {
-                // partial last chunk (entries past cnt are never read)
-                const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
-            }
-            c.ncount[t] = cnt;
-            if (cnt > K)
-            {
-                c.overflowList[atomicAdd(c.overflowCount, 1U)] = t;
-            }
-        }
-    }
-}
-
 // Density of a particle whose neighbours overflow its list: the walk again,
 // accumulating in the same order. Kept out of line (rare) so that the list
 // loop of kDensity keeps a small register footprint.
@@ -820,114 +733,203 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
 // Phase 2 of computeDensities over the neighbour list, plus
 // computeBoundaryDensityContribution; also precomputes p / rho^2. Particles
 // whose neighbours overflow the list are left to kDensityOverflow.
-__global__ __launch_bounds__(256) void kDensity(Ctx c)
+// computeDensities (Algorithm.cu:17-54) for sorted slot t over its
+// neighbour list (cnt <= listCap entries), in list order: the reference's
+// visiting order. UB and GB are the entries whose data are loaded together in
+// the uniform-h/m and general loops.
+template <uint32_t UB, uint32_t GB>
+__device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q)
 {
-    const uint32_t n = *c.nTotal;
-    const uint32_t K = c.listCap;
-    for (uint32_t t = xcdBlock() * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    float rho = 0.0F;
+    if ((sc.flags & kSceneUniformHM) != 0U)
     {
-        const uint32_t s = c.sscene[t];
-        const SceneDev& sc = c.scenes[s];
-        const uint32_t cnt = c.ncount[t];
-        if (cnt > K)
+        // every neighbour has h0 and m0: the same sum with one gather
+        // (the position) per pair
+        const float h0 = sc.h0;
+        const float m0 = sc.m0;
+        const float c3 = coefOverH3(h0);
+        uint32_t e = 0;
+        for (; e + UB <= cnt; e += UB)
         {
-            continue;  // kDensityOverflow
+            uint32_t j[UB];
+            float3 o[UB];
+#pragma unroll
+            for (uint32_t u = 0; u < UB; u += 4U)
+            {
+                const uint4 v = loadChunk(listAt(c, t, e + u));
+                j[u] = v.x;
+                j[u + 1U] = v.y;
+                j[u + 2U] = v.z;
+                j[u + 3U] = v.w;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < UB; ++u)
+            {
+                o[u] = *reinterpret_cast<const float3*>(c.s0 + j[u]);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < UB; ++u)
+            {
+                const float dx = o[u].x - q.x;
+                const float dy = o[u].y - q.y;
+                const float dz = o[u].z - q.z;
+                const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
+                rho = rho + m0 * wendlandC(d, h0, c3);
+            }
         }
+        for (; e < cnt; ++e)
+        {
+            const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, t, e));
+            const float dx = o.x - q.x;
+            const float dy = o.y - q.y;
+            const float dz = o.z - q.z;
+            rho = rho + m0 * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, c3);
+        }
+    }
+    else
+    {
+        uint32_t e = 0;
+        for (; e + GB <= cnt; e += GB)
+        {
+            uint32_t j[GB];
+            F4 o[GB];
+            F4 hm[GB];
+#pragma unroll
+            for (uint32_t u = 0; u < GB; u += 4U)
+            {
+                const uint4 v = loadChunk(listAt(c, t, e + u));
+                j[u] = v.x;
+                j[u + 1U] = v.y;
+                j[u + 2U] = v.z;
+                j[u + 3U] = v.w;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < GB; ++u)
+            {
+                o[u] = c.s0[j[u]];
+                hm[u] = c.shm[j[u]];
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < GB; ++u)
+            {
+                const float dx = o[u].x - q.x;
+                const float dy = o[u].y - q.y;
+                const float dz = o[u].z - q.z;
+                const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
+                rho = rho + hm[u].y * wendlandC(d, hm[u].x, hm[u].z);
+            }
+        }
+        for (; e < cnt; ++e)
+        {
+            const uint32_t jj = *listAt(c, t, e);
+            const F4 o = c.s0[jj];
+            const F4 hm = c.shm[jj];
+            const float dx = o.x - q.x;
+            const float dy = o.y - q.y;
+            const float dz = o.z - q.z;
+            rho = rho + hm.y * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), hm.x, hm.z);
+        }
+    }
+    return rho;
+}
+
+// Neighbour lists and densities (computeDensities, Algorithm.cu:17-54): each
+// thread builds its particle's list, then sums the density over it. BS (the
+// block size) is a compile-time constant so that the staging addresses fold
+// into immediates.
+template <uint32_t BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
+{
+    // staging ring of kRing entries per thread, entry-major ([e][thread]) so
+    // that the lanes' writes fall in distinct banks; per wave, the survivor
+    // table of the candidate filter
+    constexpr uint32_t kRing = 2U * kChunk;
+    constexpr uint32_t kTable = 64U + kStashPad;
+    __shared__ uint32_t stage[kRing * BS];
+    __shared__ F4 stash[(BS / 64U) * kTable];
+    __shared__ uint32_t slots[(BS / 64U) * kTable];
+    const uint32_t n = *c.nTotal;
+    const uint32_t nPad = (n + 63U) & ~63U;
+    const uint32_t K = c.listCap;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63U;
+    F4* wstash = stash + (tid / 64U) * kTable;
+    uint32_t* wslot = slots + (tid / 64U) * kTable;
+    // one particle per thread (the grid covers the slots; a loop here would
+    // let the compiler hoist the pressure's constants across the walk)
+    const uint32_t tw = blockIdx.x * BS + tid;
+    if (tw < nPad)
+    {
+        const bool valid = tw < n;
+        const uint32_t t = valid ? tw : n - 1U;
+        const uint32_t s = c.sscene[t];
+        const uint32_t sw = __builtin_amdgcn_readfirstlane(s);
+        const bool oneScene = __ballot((valid && s != sw) ? 1 : 0) == 0ULL;
+        const SceneDev& sc = c.scenes[s];
+        const uint32_t lost = c.states[s].lostEnd;
         const F4 q = c.s0[t];
         const float hi = c.shm[t].x;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        float rho = 0.0F;
-        if ((sc.flags & kSceneUniformHM) != 0U)
-        {
-            // every neighbour has h0 and m0: the same sum with one gather
-            // (the position) per pair
-            const float h0 = sc.h0;
-            const float m0 = sc.m0;
-            const float c3 = coefOverH3(h0);
-            uint32_t e = 0;
-            for (; e + kDensityUniformBatch <= cnt; e += kDensityUniformBatch)
+        // entries [done, cnt) are staged at ring positions (done .. cnt-1) mod
+        // kRing; at most kRing - 1 are staged, so position cnt mod kRing is free
+        uint32_t cnt = 0;
+        uint32_t done = 0;
+        // the thread's next chunk: chunk k of slot t is the 16 bytes at
+        // list + (k * pCap + t) * 4 entries
+        uint4* cptr = reinterpret_cast<uint4*>(c.list) + t;
+        const uint32_t cstep = c.pCap;
+        auto push = [&](bool hit, uint32_t j) {
+            stage[(cnt & (kRing - 1U)) * BS + tid] = j;
+            cnt += hit ? 1U : 0U;
+        };
+        // writes the oldest staged chunk once it is full (called after at most
+        // kChunk pushes, so the ring never holds more than kRing - 1 entries)
+        auto drain = [&]() {
+            if (cnt - done >= kChunk)
             {
-                uint32_t j[kDensityUniformBatch];
-                float3 o[kDensityUniformBatch];
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityUniformBatch; u += 4U)
+                if (done < K)
                 {
-                    const uint4 v = loadChunk(listAt(c, t, e + u));
-                    j[u] = v.x;
-                    j[u + 1U] = v.y;
-                    j[u + 2U] = v.z;
-                    j[u + 3U] = v.w;
+                    const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
+                    storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
+                    cptr += cstep;
                 }
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityUniformBatch; ++u)
-                {
-                    o[u] = *reinterpret_cast<const float3*>(c.s0 + j[u]);
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityUniformBatch; ++u)
-                {
-                    const float dx = o[u].x - q.x;
-                    const float dy = o[u].y - q.y;
-                    const float dz = o[u].z - q.z;
-                    const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
-                    rho = rho + m0 * wendlandC(d, h0, c3);
-                }
+                done += kChunk;
             }
-            for (; e < cnt; ++e)
+        };
+        if (oneScene)
+        {
+            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain);
+        }
+        else if (valid)
+        {
+            walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
+                push(true, j);
+                drain();
+            });
+        }
+        if (valid)
+        {
+            if (cnt <= K && cnt > done)
             {
-                const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, t, e));
-                const float dx = o.x - q.x;
-                const float dy = o.y - q.y;
-                const float dz = o.z - q.z;
-                rho = rho + m0 * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, c3);
+                // partial last chunk (entries past cnt are never read)
+                const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
+                storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
+            }
+            c.ncount[t] = cnt;
+            if (cnt > K)
+            {
+                c.overflowList[atomicAdd(c.overflowCount, 1U)] = t;  // kDensityOverflow
+            }
+            else
+            {
+                // the density over the list just written (its entries and the
+                // neighbours' records are still in cache), then the boundary
+                // term, the pressure and the force records
+                const float rho = listDensity<kListDensityUniformBatch, kListDensityBatch>(c, sc, t, cnt, q);
+                finishDensity(c, sc, b, q, hi, t, rho);
             }
         }
-        else
-        {
-            uint32_t e = 0;
-            for (; e + kDensityBatch <= cnt; e += kDensityBatch)
-            {
-                uint32_t j[kDensityBatch];
-                F4 o[kDensityBatch];
-                F4 hm[kDensityBatch];
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityBatch; u += 4U)
-                {
-                    const uint4 v = loadChunk(listAt(c, t, e + u));
-                    j[u] = v.x;
-                    j[u + 1U] = v.y;
-                    j[u + 2U] = v.z;
-                    j[u + 3U] = v.w;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityBatch; ++u)
-                {
-                    o[u] = c.s0[j[u]];
-                    hm[u] = c.shm[j[u]];
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < kDensityBatch; ++u)
-                {
-                    const float dx = o[u].x - q.x;
-                    const float dy = o[u].y - q.y;
-                    const float dz = o[u].z - q.z;
-                    const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
-                    rho = rho + hm[u].y * wendlandC(d, hm[u].x, hm[u].z);
-                }
-            }
-            for (; e < cnt; ++e)
-            {
-                const uint32_t jj = *listAt(c, t, e);
-                const F4 o = c.s0[jj];
-                const F4 hm = c.shm[jj];
-                const float dx = o.x - q.x;
-                const float dy = o.y - q.y;
-                const float dz = o.z - q.z;
-                rho = rho + hm.y * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), hm.x, hm.z);
-            }
-        }
-        finishDensity(c, sc, b, q, hi, t, rho);
     }
 }
 
@@ -2157,9 +2159,9 @@ void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
-    const uint32_t b = lc.blockNeighbor;
-    hipLaunchKernelGGL(kDensity, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    // the list builder sums the densities of the particles whose list fits
     hipLaunchKernelGGL(kDensityOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);
+    (void)lc;
 }
 
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
