@@ -26,6 +26,14 @@ __host__ __forceinline__ uint32_t gridFor(uint32_t n, uint32_t block)
     return g == 0 ? 1U : (g > kGridCap ? kGridCap : g);
 }
 
+// A grid of one thread per slot, uncapped, for kernels without a grid-stride
+// loop.
+__host__ __forceinline__ uint32_t gridCover(uint32_t n, uint32_t block)
+{
+    const uint32_t g = static_cast<uint32_t>((static_cast<uint64_t>(n) + block - 1U) / block);
+    return g == 0 ? 1U : g;
+}
+
 __device__ uint32_t blockScan256(uint32_t v, uint32_t* lds, uint32_t* total)
 {
     const int tid = static_cast<int>(threadIdx.x);
@@ -2145,15 +2153,15 @@ void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     const uint32_t b = lc.blockNeighbor;
     if (b == 64U)
     {
-        hipLaunchKernelGGL(kNeighbors<64>, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<64>, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
     }
     else if (b == 128U)
     {
-        hipLaunchKernelGGL(kNeighbors<128>, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<128>, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
     }
     else
     {
-        hipLaunchKernelGGL(kNeighbors<256>, dim3(gridFor(c.nMax, 256U)), dim3(256), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<256>, dim3(gridCover(c.nMax, 256U)), dim3(256), 0, s, c);
     }
 }
 

@@ -95,3 +95,19 @@ def test_launch_shapes_do_not_change_results():
             for s, x, y in zip(scenes, out, ref):
                 assert_bitexact(x, y, f"{s.name} block={block} radix={radix} neighbour block={nb}")
         b.close()
+
+
+def test_neighbour_grid_covers_more_slots_than_capped_grids():
+    """More particles than 16,384 blocks of 64 threads hold: the list builder
+    (one particle per thread) still reaches every slot."""
+    big = scene.cfg2()
+    small = scene.dam_break((8, 10, 8), 0.01, (0.2, 0.2, 0.2), seed=46, steps=0, name="small")
+    scenes = [big, small]
+    b = Batch.for_scenes(scenes, graph=True)
+    ids = [b.add(s) for s in scenes]
+    b.set_launch(256, 0, True, block_neighbor=64)
+    b.step(2)
+    assert b.step_particles() > 16384 * 64
+    for s, sid in zip(scenes, ids):
+        assert_bitexact(b.fluid(sid), _alone(s, 2), f"{s.name} with a neighbour block of 64")
+    b.close()
