@@ -250,6 +250,7 @@ def _declare(lib: C.CDLL) -> None:
         "asph_debug_grid": (C.c_int, [vp, vp, vp, vp, vp]),
         "asph_debug_neighbor_sets": (C.c_int, [vp, vp, vp]),
         "asph_debug_refinement": (C.c_int, [vp, P(RefinementTrace), vp, vp, vp, vp]),
+        "asph_debug_division_check": (C.c_int, [C.c_uint64, C.c_uint64, P(C.c_uint64)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)

@@ -21,7 +21,9 @@ def grid_cells(params: abi.Parameters) -> int:
     return int(n.value)
 
 
-def scene_bytes(s: Scene, list_capacity: int = 128) -> int:
+def scene_bytes(s: Scene, list_capacity: int = 0) -> int:
+    """Device bytes of scene `s` inside a batch with `list_capacity` list
+    entries per particle (0: the default of a batch of this scene alone)."""
     lib = abi.load_library()
     return int(lib.asph_scene_bytes(s.capacity, len(s.boundary), grid_cells(s.params), int(s.adaptive),
                                     list_capacity))
@@ -29,7 +31,8 @@ def scene_bytes(s: Scene, list_capacity: int = 128) -> int:
 
 class Batch:
     def __init__(self, max_particles: int, max_boundary: int, max_cells: int, max_scenes: int,
-                 adaptive: bool = False, graph: bool = True, list_capacity: int = 128):
+                 adaptive: bool = False, graph: bool = True, list_capacity: int = 0):
+        # list_capacity 0: the library default (128, or 640 for adaptive batches)
         self._lib = abi.load_library()
         cfg = abi.BatchConfig()
         cfg.max_particles = int(max_particles)

@@ -176,6 +176,7 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
         cfg.boundaryCapacity = max_boundary_capacity;
         cfg.cellCapacity = g.cells() + 1U;
         cfg.sceneSlots = 1;
+        cfg.listCap = asph::defaultListCap(adaptive);
         cfg.adaptive = adaptive;
         cfg.useGraph = false;
         auto sim = std::make_unique<asph_simulation>();
@@ -325,6 +326,29 @@ int asph_debug_refinement(asph_simulation* sim, asph_refinement_trace* trace, ui
     });
 }
 
+int asph_debug_division_check(uint64_t seed, uint64_t count, uint64_t* mismatches)
+{
+    return guard([&] {
+        need(mismatches, "mismatches");
+        unsigned long long* bad = nullptr;
+        asph::hipCheck(hipMalloc(&bad, sizeof(*bad)), "division check");
+        unsigned long long h = 0;
+        hipError_t e = hipMemset(bad, 0, sizeof(*bad));
+        if (e == hipSuccess)
+        {
+            asph::launchDivisionCheck(seed, count, bad, nullptr);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess)
+        {
+            e = hipMemcpy(&h, bad, sizeof(h), hipMemcpyDeviceToHost);
+        }
+        (void)hipFree(bad);
+        asph::hipCheck(e, "division check");
+        *mismatches = h;
+    });
+}
+
 // ------------------------------------------------------------------ batch --
 
 int asph_batch_create(const asph_batch_config* config, asph_batch** out)
@@ -338,8 +362,8 @@ int asph_batch_create(const asph_batch_config* config, asph_batch** out)
         cfg.boundaryCapacity = config->max_boundary;
         cfg.cellCapacity = config->max_cells;
         cfg.sceneSlots = config->max_scenes == 0 ? 1U : config->max_scenes;
-        cfg.listCap = config->list_capacity == 0 ? 128U : config->list_capacity;
         cfg.adaptive = (config->flags & ASPH_BATCH_ADAPTIVE) != 0U;
+        cfg.listCap = config->list_capacity == 0 ? asph::defaultListCap(cfg.adaptive) : config->list_capacity;
         cfg.useGraph = (config->flags & ASPH_BATCH_GRAPH) != 0U;
         auto b = std::make_unique<asph_batch>();
         b->engine = std::make_unique<asph::Engine>(cfg);
@@ -482,7 +506,7 @@ uint64_t asph_scene_bytes(uint32_t capacity, uint32_t boundary_count, uint32_t c
                           uint32_t list_capacity)
 {
     return asph::sceneBytes(capacity, boundary_count, cells, adaptive != 0U,
-                            list_capacity == 0 ? 128U : list_capacity);
+                            list_capacity == 0 ? asph::defaultListCap(adaptive != 0U) : list_capacity);
 }
 
 int asph_grid_cells(const asph_domain* domain, float smoothing_radius, uint32_t* cells)

@@ -5,7 +5,9 @@
 // IEEE-correct FP32 division / square root): each expression follows the
 // reference statement it restates in the same association order, so cell
 // keys, neighbour sets, densities, forces and trajectories are bit-identical
-// to the host restatement run in FP32.
+// to the host restatement run in FP32. Quotients that share a divisor go
+// through one reciprocal (divRecip / divBy) where that yields the quotient's
+// exact bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -138,19 +140,6 @@ __device__ __forceinline__ float wendland(float d, float h)
     return ((kWCoef / h3) * tmp4) * (2.0F * q + 1.0F);
 }
 
-// wendland with (coef / h^3) precomputed per particle (same rounding)
-__device__ __forceinline__ float wendlandC(float d, float h, float coefOverH3)
-{
-    const float q = d / h;
-    if (q > 2.0F)
-    {
-        return 0.0F;
-    }
-    const float tmp = 1.0F - 0.5F * q;
-    const float tmp4 = (tmp * tmp) * (tmp * tmp);
-    return (coefOverH3 * tmp4) * (2.0F * q + 1.0F);
-}
-
 // wendlandDerivativeKernel (WendlandKernel.cuh:69-84)
 __device__ __forceinline__ float wendlandGrad(float d, float h)
 {
@@ -166,10 +155,51 @@ __device__ __forceinline__ float wendlandGrad(float d, float h)
     return ((kWDCoef / h4) * q) * tmp3;
 }
 
-// wendlandGrad with (dcoef / h^4) precomputed per particle
-__device__ __forceinline__ float wendlandGradC(float d, float h, float dcoefOverH4)
+// ---- division by a shared divisor, same bits as the IEEE quotient ----------
+// The correctly rounded FP32 division a / b that hipcc emits for gfx950 is:
+// range scaling of a and b (v_div_scale), r = rcp(b), e = fma(-b, r, 1),
+// y = fma(e, r, r), q0 = a * y, r0 = fma(-b, q0, a), q1 = fma(r0, y, q0),
+// r1 = fma(-b, q1, a), fma(r1, y, q1) with a scale fix-up (v_div_fmas), and a
+// special-value fix-up (v_div_fixup). When a is +0 or |a|, |b| lie in
+// [2^-40, 2^40] the scalings are the identity (no operand or quotient near
+// the denormal or overflow range), v_div_fmas is a plain fma and v_div_fixup
+// returns its input, so divBy gives the quotient's exact bits while y, which
+// depends on b only, is shared by every numerator over b.
+constexpr float kDivLo = 0x1p-40F;
+constexpr float kDivHi = 0x1p40F;
+
+__device__ __forceinline__ float divRecip(float b)
 {
-    const float q = d / h;
+    const float r = __builtin_amdgcn_rcpf(b);
+    const float e = __builtin_fmaf(-b, r, 1.0F);
+    return __builtin_fmaf(e, r, r);
+}
+
+__device__ __forceinline__ float divBy(float a, float b, float y)
+{
+    const float q0 = a * y;
+    const float r0 = __builtin_fmaf(-b, q0, a);
+    const float q1 = __builtin_fmaf(r0, y, q0);
+    const float r1 = __builtin_fmaf(-b, q1, a);
+    return __builtin_fmaf(r1, y, q1);
+}
+
+// wendlandKernel / wendlandDerivativeKernel (WendlandKernel.cuh:39-52, 69-84)
+// given q = d / h and the per-particle factor; wendlandC and wendlandGradC
+// below are these with q = d / h
+__device__ __forceinline__ float wendlandQ(float q, float coefOverH3)
+{
+    if (q > 2.0F)
+    {
+        return 0.0F;
+    }
+    const float tmp = 1.0F - 0.5F * q;
+    const float tmp4 = (tmp * tmp) * (tmp * tmp);
+    return (coefOverH3 * tmp4) * (2.0F * q + 1.0F);
+}
+
+__device__ __forceinline__ float wendlandGradQ(float q, float dcoefOverH4)
+{
     if (q > 2.0F)
     {
         return 0.0F;
@@ -177,6 +207,18 @@ __device__ __forceinline__ float wendlandGradC(float d, float h, float dcoefOver
     const float tmp = 1.0F - 0.5F * q;
     const float tmp3 = (tmp * tmp) * tmp;
     return (dcoefOverH4 * q) * tmp3;
+}
+
+// wendland with (coef / h^3) precomputed per particle (same rounding)
+__device__ __forceinline__ float wendlandC(float d, float h, float coefOverH3)
+{
+    return wendlandQ(d / h, coefOverH3);
+}
+
+// wendlandGrad with (dcoef / h^4) precomputed per particle
+__device__ __forceinline__ float wendlandGradC(float d, float h, float dcoefOverH4)
+{
+    return wendlandGradQ(d / h, dcoefOverH4);
 }
 
 __device__ __forceinline__ float coefOverH3(float h) { return kWCoef / ((h * h) * h); }

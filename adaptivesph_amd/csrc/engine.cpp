@@ -115,6 +115,15 @@ namespace
 inline float hmin(float a, float b) { return (b < a) ? b : a; }
 inline float hmax(float a, float b) { return (a < b) ? b : a; }
 
+// |v| in [2^-40, 2^40] (kDivLo, kDivHi of device_common.hpp): a divisor the
+// device may divide by through its reciprocal with the quotient's exact bits
+inline bool sharedDivisor(float v)
+{
+    uint32_t u;
+    std::memcpy(&u, &v, sizeof(u));
+    return ((u & 0x7FFFFFFFU) - 0x2B800000U) <= 0x28000000U;
+}
+
 // calculateCellIndex + flattenCellIndex (NeighborGrid.cuh:166-177) on the host.
 int32_t hostCellKey(const SceneDev& sc, float px, float py, float pz)
 {
@@ -267,7 +276,8 @@ Engine::Engine(const EngineConfig& cfg)
         c.overflowCount = alloc<uint32_t>(1);
         cfg_.listCap = listCapacityRounded(cfg_.listCap);
         c.listCap = cfg_.listCap;
-        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap) * P);
+        // wave-blocked rows: whole waves of 64 slots
+        c.list = alloc<uint32_t>(static_cast<size_t>(cfg_.listCap) * ((static_cast<size_t>(P) + 63U) / 64U * 64U));
         c.bs0 = alloc<F4>(B);
         c.bvisc = alloc<float>(B);
         c.bcellOff = alloc<uint32_t>(static_cast<size_t>(C) + 1);
@@ -534,7 +544,10 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     d.flags = kSceneActive | (adaptive ? kSceneAdaptive : 0U) |
               ((p.flags & ASPH_FLAG_FIX_CELL_START) != 0U ? kSceneFixCellStart : 0U) |
               ((p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? kSceneKernelMergedH : 0U) |
-              (adaptive ? 0U : kSceneUniformHM);  // refinement is the only writer of h and m
+              // refinement is the only writer of h and m; the uniform loops
+              // divide by h0 through its reciprocal, valid for h0 in
+              // [2^-40, 2^40]
+              (adaptive || !sharedDivisor(p.base_smoothing_radius) ? 0U : kSceneUniformHM);
     d.tpb = p.threads_per_block;
     if (r != nullptr)
     {
@@ -1327,7 +1340,7 @@ void Engine::setFluid(uint32_t id, uint32_t count, const asph_vec4* pos, const a
     };
     hs.knownH = known(h, hs.dev.h0, hs.knownH);
     hs.knownM = known(m, hs.dev.m0, hs.knownM);
-    const bool uniform = !hs.adaptive && hs.knownH >= count && hs.knownM >= count;
+    const bool uniform = !hs.adaptive && hs.knownH >= count && hs.knownM >= count && sharedDivisor(hs.dev.h0);
     const uint32_t flags = uniform ? (hs.dev.flags | kSceneUniformHM) : (hs.dev.flags & ~kSceneUniformHM);
     if (flags != hs.dev.flags)
     {

@@ -59,6 +59,15 @@ GridGeom makeGrid(const asph_domain& d, float h0);
 // Throws Error(ASPH_ERR_INVALID) for parameters the pipeline cannot run.
 void validateParams(const asph_parameters& p);
 
+// Default neighbour-list entries per particle: plain scenes hold about 80
+// neighbours (h = 1.3 dx); refinement packs up to 13 daughters per parent
+// volume, and refined regions hold several hundred neighbours per particle.
+// Particles beyond the capacity take the slower walking overflow kernels.
+constexpr uint32_t kDefaultListCapPlain = 128;
+constexpr uint32_t kDefaultListCapAdaptive = 640;
+
+inline uint32_t defaultListCap(bool adaptive) { return adaptive ? kDefaultListCapAdaptive : kDefaultListCapPlain; }
+
 struct EngineConfig
 {
     uint32_t particleCapacity = 0;  // pool of fluid particle slots

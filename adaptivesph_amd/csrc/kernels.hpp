@@ -82,7 +82,8 @@ struct Ctx
     // number (reset by kPrep each step)
     uint32_t* overflowList;
     uint32_t* overflowCount;
-    uint32_t* list;  // [listCap * pCap] in 4-entry chunks: ((k / 4) * pCap + t) * 4 + k % 4
+    uint32_t* list;  // [listCap * pCap rounded up to 64], 4-entry chunks, wave-blocked: entry k of slot t
+                     // at (((t / 64) * (listCap / 4) + k / 4) * 64 + t % 64) * 4 + k % 4
     uint32_t listCap;  // a multiple of 4
     // per aligned group of kGroup sorted slots: bounding box and max (2h)^2
     F4* gbox0;  // (min x, min y, min z, max (2h)^2)
@@ -159,6 +160,10 @@ void launchNeighborSets(const Ctx& c, uint32_t scene, uint32_t count, uint32_t* 
                         uint32_t* total, hipStream_t s);
 void launchDensityInfo(const Ctx& c, uint32_t scene, uint32_t count, float threshold, double* partials,
                        uint32_t* upartials, hipStream_t s);
+
+// Adds to *bad the pairs among `count` pseudo-random ones where the
+// shared-reciprocal division differs from a / b (0 expected).
+void launchDivisionCheck(uint64_t seed, uint64_t count, unsigned long long* bad, hipStream_t s);
 constexpr uint32_t kInfoBlocks = 256;
 
 }  // namespace asph

@@ -461,14 +461,18 @@ typedef const __attribute__((address_space(4))) uint32_t ConstU32;
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
 // XCD i mod 8), and each XCD has its own L2. The neighbour-list consumers
 // renumber their workgroups so that each XCD takes one contiguous eighth of
-// the sorted particles: neighbouring particles (which gather the same
-// neighbour data) then share an L2. Results do not depend on the numbering.
-// (kNeighbors reads its candidates through the scalar cache and measured
-// faster without the renumbering.)
-__device__ __forceinline__ uint32_t xcdBlock()
+// the n sorted particles: neighbouring particles (which gather the same
+// neighbour data) then share an L2. The eighths are taken of the workgroups
+// that have particles (the grid is sized for the pool's capacity, which can
+// be many times n). Results do not depend on the numbering. (kNeighbors
+// reads its candidates through the scalar cache and measured faster without
+// the renumbering.)
+__device__ __forceinline__ uint32_t xcdBlock(uint32_t n)
 {
     constexpr uint32_t kXcds = 8;
-    const uint32_t per = gridDim.x / kXcds;
+    const uint32_t needed = (n + blockDim.x - 1U) / blockDim.x;
+    const uint32_t used = needed < gridDim.x ? needed : gridDim.x;
+    const uint32_t per = used / kXcds;
     const uint32_t b = blockIdx.x;
     if (b >= per * kXcds)
     {
@@ -526,9 +530,19 @@ __device__ __forceinline__ void storeChunk(uint4* p, uint32_t a, uint32_t b, uin
     *p = uint4 {a, b, c, d};
 }
 
+// Wave-blocked list layout: the 64 slots of an aligned wave (t & ~63) own
+// listCap * 64 consecutive entries, chunk row by chunk row; chunk k / 4 of
+// slot t is the 16 bytes at chunk index ((t / 64) * (listCap / 4) + k / 4) * 64
+// + t % 64. A wave reads or writes 1 KiB contiguously per chunk row, and its
+// whole list is one contiguous block (few pages per wave, whatever the pool).
+__device__ __forceinline__ size_t listChunk(const Ctx& c, uint32_t t, uint32_t k)
+{
+    return (static_cast<size_t>(t >> 6) * (c.listCap / kChunk) + k / kChunk) * 64U + (t & 63U);
+}
+
 __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint32_t k)
 {
-    return c.list + (static_cast<size_t>(k / kChunk) * c.pCap + t) * kChunk + (k % kChunk);
+    return c.list + listChunk(c, t, k) * kChunk + (k % kChunk);
 }
 
 __device__ __forceinline__ float waveMinF(float v)
@@ -738,6 +752,18 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
     c.fb[t] = F4 {v.x, v.y, v.z, rho};
 }
 
+// d / h0 for a distance d >= +0 (a square root) through y0 = divRecip(h0),
+// h0 in [2^-40, 2^40]; the division where d is outside +0 or [2^-40, 2^40]
+__device__ __forceinline__ float quotientH0(float d, float h0, float y0)
+{
+    float q = divBy(d, h0, y0);
+    if (!(d <= kDivHi && (d >= kDivLo || d == 0.0F)))
+    {
+        q = d / h0;
+    }
+    return q;
+}
+
 // Phase 2 of computeDensities over the neighbour list, plus
 // computeBoundaryDensityContribution; also precomputes p / rho^2. Particles
 // whose neighbours overflow the list are left to kDensityOverflow.
@@ -752,10 +778,12 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
     if ((sc.flags & kSceneUniformHM) != 0U)
     {
         // every neighbour has h0 and m0: the same sum with one gather
-        // (the position) per pair
+        // (the position) per pair; h0 lies in [2^-40, 2^40] (a condition of
+        // the flag), so d / h0 goes through its reciprocal
         const float h0 = sc.h0;
         const float m0 = sc.m0;
         const float c3 = coefOverH3(h0);
+        const float y0 = divRecip(h0);
         uint32_t e = 0;
         for (; e + UB <= cnt; e += UB)
         {
@@ -782,7 +810,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
                 const float dy = o[u].y - q.y;
                 const float dz = o[u].z - q.z;
                 const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
-                rho = rho + m0 * wendlandC(d, h0, c3);
+                rho = rho + m0 * wendlandQ(quotientH0(d, h0, y0), c3);
             }
         }
         for (; e < cnt; ++e)
@@ -791,7 +819,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
             const float dx = o.x - q.x;
             const float dy = o.y - q.y;
             const float dz = o.z - q.z;
-            rho = rho + m0 * wendlandC(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, c3);
+            rho = rho + m0 * wendlandQ(quotientH0(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, y0), c3);
         }
     }
     else
@@ -883,10 +911,10 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         // kRing; at most kRing - 1 are staged, so position cnt mod kRing is free
         uint32_t cnt = 0;
         uint32_t done = 0;
-        // the thread's next chunk: chunk k of slot t is the 16 bytes at
-        // list + (k * pCap + t) * 4 entries
-        uint4* cptr = reinterpret_cast<uint4*>(c.list) + t;
-        const uint32_t cstep = c.pCap;
+        // the thread's next chunk (listChunk): chunk rows of a wave's block
+        // are 64 chunks apart
+        uint4* cptr = reinterpret_cast<uint4*>(c.list) + listChunk(c, t, 0U);
+        constexpr uint32_t cstep = 64U;
         auto push = [&](bool hit, uint32_t j) {
             stage[(cnt & (kRing - 1U)) * BS + tid] = j;
             cnt += hit ? 1U : 0U;
@@ -972,10 +1000,16 @@ struct PairAcc
 // computeSurfaceTensionAccelerations (:192-218), accumulated separately.
 // fa = (position, p/rho^2) and fb = (velocity, rho) of neighbour j; mj, hj,
 // wdj = its mass, smoothing length and kWDCoef / h^4.
+// With kSharedH, yh = divRecip(hj) and hj lies in [2^-40, 2^40], so d / hj
+// is taken through the shared reciprocal (same bits as the division).
+template <bool kSharedH = false>
 __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, float hj, float wdj, const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
-                                          bool doST, PairAcc& A)
+                                          bool doST, PairAcc& A, float yh = 0.0F, bool use = true)
 {
+    // use == false (the particle itself) computes and discards: the pair is
+    // evaluated unconditionally so that the callers' gathers are not sunk
+    // into a branch (which serialises their latency)
     const F4& o = fa;
     const float ox = q.x - o.x;
     const float oy = q.y - o.y;
@@ -985,24 +1019,44 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
     float dx = 0.0F;
     float dy = 1.0F;
     float dz = 0.0F;
-    if (d > 0.0F)
+    float qh;
+    // every component in [2^-40, 2^39] puts d in [2^-40, 2^40] too: then the
+    // four quotients (three over d, one over hj) go through reciprocals
+    const float lo = fminf(fminf(fabsf(ox), fabsf(oy)), fabsf(oz));
+    const float hi = fmaxf(fmaxf(fabsf(ox), fabsf(oy)), fabsf(oz));
+    if ((lo >= kDivLo && hi <= 0x1p39F) || !use)
     {
-        dx = ox / d;
-        dy = oy / d;
-        dz = oz / d;
+        const float yd = divRecip(d);
+        dx = divBy(ox, d, yd);
+        dy = divBy(oy, d, yd);
+        dz = divBy(oz, d, yd);
+        qh = kSharedH ? divBy(d, hj, yh) : d / hj;
+    }
+    else
+    {
+        if (d > 0.0F)
+        {
+            dx = ox / d;
+            dy = oy / d;
+            dz = oz / d;
+        }
+        qh = d / hj;
     }
     const float pterm = prri + fa.w;
-    const float dw = wendlandGradC(d, hj, wdj);
-    A.apx = A.apx - ((mj * dx) * dw) * pterm;
-    A.apy = A.apy - ((mj * dy) * dw) * pterm;
-    A.apz = A.apz - ((mj * dz) * dw) * pterm;
+    const float dw = wendlandGradQ(qh, wdj);
+    const float px = A.apx - ((mj * dx) * dw) * pterm;
+    const float py = A.apy - ((mj * dy) * dw) * pterm;
+    const float pz = A.apz - ((mj * dz) * dw) * pterm;
+    A.apx = use ? px : A.apx;
+    A.apy = use ? py : A.apy;
+    A.apz = use ? pz : A.apz;
     if (doVisc)
     {
         const float vx = vi.x - fb.x;
         const float vy = vi.y - fb.y;
         const float vz = vi.z - fb.z;
         const float comp = dot3w(vx, vy, vz, ox, oy, oz);
-        if (comp < 0.0F)
+        if (use && comp < 0.0F)
         {
             const float nu = nuNum / (rhoi + fb.w);
             const float piv = ((-nu) * comp) / (d2 + epsH2);
@@ -1012,7 +1066,7 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
             A.avz = A.avz - sv * dz;
         }
     }
-    if (doST)
+    if (doST && use)
     {
         const float sw = mj * wendland(d, hj);
         A.asx = A.asx + sw * dx;
@@ -1198,7 +1252,9 @@ __device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
 // Fluid pair sums of slot t over its neighbour list, in list order. With
 // kUniform every neighbour has h0 and m0 (two gathers per pair), otherwise
 // the mass and smoothing length are gathered too.
-template <bool kUniform>
+// With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^40]) the quotient
+// d / h0 is taken through the reciprocal of h0.
+template <bool kUniform, bool kSharedH = false>
 __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
                                           bool doST, PairAcc& A)
@@ -1206,6 +1262,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     const float m0 = sc.m0;
     const float h0 = sc.h0;
     const float wd0 = kUniform ? dcoefOverH4(h0) : 0.0F;
+    const float y0 = kSharedH ? divRecip(h0) : 0.0F;
     constexpr uint32_t B = kUniform ? kForceUniformBatch : kForceBatch;
     uint32_t e = 0;
     for (; e + B <= cnt; e += B)
@@ -1236,11 +1293,9 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u)
         {
-            if (j[u] != t)
-            {
-                forcePair(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x, kUniform ? wd0 : hm[u].w,
-                          q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
-            }
+            forcePair<kSharedH>(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x,
+                                kUniform ? wd0 : hm[u].w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A, y0,
+                                j[u] != t);
         }
     }
     for (; e < cnt; ++e)
@@ -1250,7 +1305,8 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
         {
             if (kUniform)
             {
-                forcePair(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+                forcePair<kSharedH>(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A,
+                                    y0);
             }
             else
             {
@@ -1261,11 +1317,11 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     }
 }
 
-__global__ __launch_bounds__(256) void kForce(Ctx c)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kForce(Ctx c)
 {
     const uint32_t n = *c.nTotal;
     const uint32_t K = c.listCap;
-    for (uint32_t t = xcdBlock() * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    for (uint32_t t = xcdBlock(n) * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
         const uint32_t cnt = c.ncount[t];
         if (cnt > K)
@@ -1289,7 +1345,8 @@ __global__ __launch_bounds__(256) void kForce(Ctx c)
         PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
         if ((sc.flags & kSceneUniformHM) != 0U)
         {
-            forceList<true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            // (the flag is set only for h0 in [2^-40, 2^40])
+            forceList<true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
         }
         else
         {
@@ -2256,6 +2313,67 @@ void launchDensityInfo(const Ctx& c, uint32_t scene, uint32_t count, float thres
 {
     hipLaunchKernelGGL(kDensityInfo, dim3(kInfoBlocks), dim3(256), 0, s, c, scene, count, threshold, partials,
                        upartials);
+}
+
+// ------------------------------------------------- division self-check --
+// Compares divBy(a, b, divRecip(b)) with the division a / b on `count`
+// pseudo-random pairs: a = +0 or +-m * 2^ea, b = m' * 2^eb with exponents
+// uniform in [-40, 39] and random 23-bit mantissas, plus, every fourth pair,
+// a in [0, 2.2 b] (the kernels' d / h range). Counts the pairs whose bits differ.
+// SplitMix64 finaliser
+__device__ __forceinline__ uint64_t checkMix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+__device__ __forceinline__ float rangedFloat(uint64_t r)
+{
+    const uint32_t e = static_cast<uint32_t>((r >> 23) % 80U) + 87U;  // 2^-40 .. 2^39
+    return __uint_as_float((e << 23) | static_cast<uint32_t>(r & 0x7FFFFFU));
+}
+
+__global__ __launch_bounds__(256) void kDivisionCheck(uint64_t seed, uint64_t count, unsigned long long* bad)
+{
+    uint32_t local = 0;
+    for (uint64_t k = blockIdx.x * 256ULL + threadIdx.x; k < count; k += static_cast<uint64_t>(gridDim.x) * 256ULL)
+    {
+        const uint64_t r0 = checkMix(seed ^ (k * 2ULL));
+        const uint64_t r1 = checkMix(seed ^ (k * 2ULL + 1ULL));
+        const float b = rangedFloat(r1);
+        float a;
+        if ((k & 3ULL) == 3ULL)
+        {
+            a = b * (2.2F * static_cast<float>(r0 >> 40) * 0x1p-24F);  // d / h range, includes +0
+        }
+        else if ((k & 255ULL) == 254ULL)
+        {
+            a = 0.0F;
+        }
+        else
+        {
+            a = rangedFloat(r0);
+            a = ((r0 >> 63) != 0ULL) ? -a : a;
+        }
+        if (!(a == 0.0F || (fabsf(a) >= kDivLo && fabsf(a) <= kDivHi)))
+        {
+            continue;  // outside the domain of divBy (tiny products of the d / h case)
+        }
+        const float fast = divBy(a, b, divRecip(b));
+        const float ieee = a / b;
+        local += (__float_as_uint(fast) != __float_as_uint(ieee)) ? 1U : 0U;
+    }
+    if (local != 0U)
+    {
+        atomicAdd(bad, static_cast<unsigned long long>(local));
+    }
+}
+
+void launchDivisionCheck(uint64_t seed, uint64_t count, unsigned long long* bad, hipStream_t s)
+{
+    hipLaunchKernelGGL(kDivisionCheck, dim3(4096), dim3(256), 0, s, seed, count, bad);
 }
 
 }  // namespace asph

@@ -59,7 +59,7 @@ def percentile(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
-def unit_bytes(list_capacity: int = 128, adaptive: bool = True) -> tuple[int, int, int, int]:
+def unit_bytes(list_capacity: int = 0, adaptive: bool = True) -> tuple[int, int, int, int]:
     """Device bytes of one fluid slot, one boundary slot and one grid cell of a
     scene inside a batch, and the fixed per-scene part (from the library)."""
     lib = abi.load_library()
@@ -71,7 +71,7 @@ def unit_bytes(list_capacity: int = 128, adaptive: bool = True) -> tuple[int, in
     return fp, fb, fc, base
 
 
-def pools_for_budget(budget_bytes: int, list_capacity: int = 128, adaptive: bool = True,
+def pools_for_budget(budget_bytes: int, list_capacity: int = 0, adaptive: bool = True,
                      boundary_ratio: float = 0.5, cell_ratio: float = 0.25) -> tuple[int, int, int]:
     """Pool sizes (fluid slots, boundary slots, cells) whose device footprint
     fits `budget_bytes`, with boundary and cell pools sized per fluid slot."""
@@ -120,7 +120,7 @@ class StepClock:
 class Server:
     """Continuous batching of scenes in one Batch against a device-memory budget."""
 
-    def __init__(self, budget_bytes: int, max_scenes: int = 256, list_capacity: int = 128,
+    def __init__(self, budget_bytes: int, max_scenes: int = 256, list_capacity: int = 0,
                  boundary_ratio: float = 0.5, cell_ratio: float = 0.25, keep_results: bool = False,
                  workers: int = 4, graph: bool = True, pools: tuple[int, int, int] | None = None,
                  batch=None):

@@ -329,14 +329,15 @@ def main() -> int:
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget")
     ap.add_argument("--list-capacity", type=int, default=0,
-                    help="neighbour-list entries per particle (0: 128 for plain workloads, 384 with refinement)")
+                    help="neighbour-list entries per particle (0: the library default, 128 for plain "
+                         "workloads, 640 with refinement)")
     ap.add_argument("--merged-h", choices=("kernel", "reference"), default="kernel",
                     help="merged smoothing length of adaptive scenes: kernel-normalised or the reference's")
     args = ap.parse_args()
     if args.list_capacity == 0:
-        # refined regions hold up to ~13x denser daughters: particles there have
-        # a few hundred neighbours
-        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy") else 384
+        # the library's defaults: refined regions hold up to ~13x denser
+        # daughters, and particles there have several hundred neighbours
+        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy") else 640
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build steps scenes on one "
               "GPU and leaves multi-GPU sharding out", file=sys.stderr)

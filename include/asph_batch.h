@@ -24,7 +24,8 @@ typedef struct asph_batch_config {
     uint32_t max_boundary;  /* boundary particle pool */
     uint32_t max_cells;     /* grid cell pool (sum over scenes of cells + 1) */
     uint32_t max_scenes;    /* scene table size */
-    uint32_t list_capacity; /* neighbour-list entries per particle (0 = 128) */
+    uint32_t list_capacity; /* neighbour-list entries per particle (0 = 128, or 640
+                               with ASPH_BATCH_ADAPTIVE) */
     uint32_t flags;         /* ASPH_BATCH_* */
 } asph_batch_config;
 
@@ -73,7 +74,8 @@ int asph_batch_memory(asph_batch* batch, uint64_t* allocated, uint64_t* particle
 int asph_batch_profile_step(asph_batch* batch, float* ms, uint32_t capacity, uint32_t* count);
 const char* asph_profile_stage_name(uint32_t stage);
 
-/* Device bytes one scene of this shape occupies inside a batch. */
+/* Device bytes one scene of this shape occupies inside a batch
+   (list_capacity 0: the batch default for `adaptive`). */
 uint64_t asph_scene_bytes(uint32_t capacity, uint32_t boundary_count, uint32_t cells, uint32_t adaptive,
                           uint32_t list_capacity);
 

@@ -61,6 +61,12 @@ int asph_debug_refinement(asph_simulation* sim, asph_refinement_trace* trace,
                           uint32_t* merge_list, uint32_t* merge_candidates,
                           uint32_t* pair_keep_ids, uint32_t* split_parents);
 
+/* Checks, on `count` pseudo-random operand pairs (seeded) with magnitudes in
+   [2^-40, 2^40], that the kernels' division through a shared reciprocal gives
+   the same bits as the FP32 division; *mismatches receives the number of
+   pairs that differ (0 expected). Uses the current device; synchronous. */
+int asph_debug_division_check(uint64_t seed, uint64_t count, uint64_t* mismatches);
+
 #ifdef __cplusplus
 }
 #endif

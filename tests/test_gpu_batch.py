@@ -111,3 +111,18 @@ def test_neighbour_grid_covers_more_slots_than_capped_grids():
     for s, sid in zip(scenes, ids):
         assert_bitexact(b.fluid(sid), _alone(s, 2), f"{s.name} with a neighbour block of 64")
     b.close()
+
+
+@pytest.mark.parametrize("list_capacity", [8, 64])
+def test_list_overflow_paths_match_alone(list_capacity):
+    """A list capacity far below the neighbour counts sends particles to the
+    overflow kernels (the walk again); results stay those of the scenes run
+    alone with the default capacity."""
+    scenes = _mix()
+    b = Batch.for_scenes(scenes, graph=True, list_capacity=list_capacity)
+    ids = [b.add(s) for s in scenes]
+    steps = 12
+    b.step(steps)
+    for s, sid in zip(scenes, ids):
+        assert_bitexact(b.fluid(sid), _alone(s, steps), f"{s.name} list capacity {list_capacity}")
+    b.close()

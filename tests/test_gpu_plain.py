@@ -270,3 +270,16 @@ def test_box_filled_with_active_walls():
     assert_bitexact(a, ref.fluid(), "box filled")
     rho0 = s.params.rest_density
     assert np.isfinite(a["densities"]).all() and a["densities"].max() > 0.5 * rho0
+
+
+def test_shared_reciprocal_division_equals_division():
+    """The density and force loops divide by a shared reciprocal (the steps
+    of the correctly rounded division without its range scaling); on 2^30
+    pseudo-random operand pairs with magnitudes in [2^-40, 2^40], plus +0 and
+    the d / h range, every quotient has the bits of a / b."""
+    import ctypes as C
+
+    lib = abi.load_library()
+    bad = C.c_uint64(123)
+    assert lib.asph_debug_division_check(7, 1 << 30, C.byref(bad)) == 0, abi.last_error()
+    assert bad.value == 0
