@@ -201,8 +201,8 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
-        // 16, compaction copies 64, merge list capture 4
-        perParticle += 24ULL + 16ULL + 64ULL + 4ULL;
+        // 16, compaction copies 64, merge list capture 4, sorted slot 4
+        perParticle += 24ULL + 16ULL + 64ULL + 4ULL + 4ULL;
     }
     // boundary slot: position + psi 16, viscosity 4, chunk box 32; cell: count,
     // offset and boundary offset 12, boundary box 32, wall distance bound 4.
@@ -293,6 +293,7 @@ Engine::Engine(const EngineConfig& cfg)
             c.ckeysB = alloc<uint64_t>(P);
             c.cvalsB = alloc<uint32_t>(P);
             c.cand = alloc<uint32_t>(P);
+            c.slotOf = alloc<uint32_t>(P);
             c.removal = alloc<uint32_t>(P);
             c.keep = alloc<uint32_t>(P);
             c.rprefix = alloc<uint32_t>(P);

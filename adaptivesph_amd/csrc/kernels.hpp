@@ -111,6 +111,7 @@ struct Ctx
     uint64_t* csortedKeys;
     uint32_t* csortedVals;
     uint32_t* cand;
+    uint32_t* slotOf;  // [pCap] sorted slot of each canonical particle (written by kGather; adaptive engines)
     uint32_t* removal;
     uint32_t* keep;
     uint32_t* rprefix;

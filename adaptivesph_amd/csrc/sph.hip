@@ -304,7 +304,12 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
             const float h3 = (hh * hh) * hh;
             const float h4 = h3 * hh;
             c.shm[t] = F4 {hh, mm, kWCoef / h3, kWDCoef / h4};
-            c.sscene[t] = c.cellOrder[k];
+            const uint32_t s = c.cellOrder[k];
+            c.sscene[t] = s;
+            if ((c.scenes[s].flags & kSceneAdaptive) != 0U)
+            {
+                c.slotOf[i] = t;  // the merge search reads the particle's neighbour list
+            }
             lx = hx = p.x;
             ly = hy = p.y;
             lz = hz = p.z;
@@ -1722,12 +1727,14 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
         const float massLimit = sc.maxMassRatio * sc.m0;
         float closest = FLT_MAX;
         uint32_t bestSlot = UINT32_MAX;
-        // The walk reads the sorted shadows, which hold the candidates'
+        // The search reads the sorted shadows, which hold the candidates'
         // current positions, smoothing lengths and masses (nothing changes
         // them between the gather and the merge), in the reference's order.
         // d2 < hmin^2 implies d2 < h_i^2, so the shadow's h and m are read only
         // past that test, and the candidate's identity only on acceptance.
-        walkSlots(c, sc, st.lostEnd, q.x, q.y, q.z, hh, [&](uint32_t slot) {
+        const uint32_t ti = c.slotOf[i];
+        // d2 of slot when it is a closer admissible candidate, else -1
+        auto visit = [&](uint32_t slot) {
             const F4 o = c.s0[slot];
             const float ox = o.x - q.x;
             const float oy = o.y - q.y;
@@ -1737,13 +1744,51 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
             {
                 const F4 hm = c.shm[slot];
                 const float hmin = gmin(hh, hm.x);
-                if (d2 < hmin * hmin && d2 < closest && hm.y < massLimit && c.sortedVals[slot] != i)
+                if (d2 < hmin * hmin && d2 < closest && hm.y < massLimit && slot != ti)
+                {
+                    return d2;
+                }
+            }
+            return -1.0F;
+        };
+        const uint32_t cnt = c.ncount[ti];
+        if (cnt <= c.listCap)
+        {
+            // Every hit of the walk over the cells within h_i is in the
+            // particle's neighbour list of this step (d2 < h_j^2 < (2 h_j)^2,
+            // cells within h_i inside those within 2 h_i), and the list holds
+            // the walk order (cells x, y, z, ascending slot) of a larger cell
+            // box: the list, restricted to the cells within h_i, is the walk.
+            const CellBox b = cellBox(sc, q.x, q.y, q.z, hh);
+            for (uint32_t e = 0; e < cnt; ++e)
+            {
+                const uint32_t slot = *listAt(c, ti, e);
+                const float d2 = visit(slot);
+                if (d2 >= 0.0F)
+                {
+                    const int local = static_cast<int>(c.sortedKeys[slot] - sc.cellBase);
+                    const int x = local % sc.gx;
+                    const int y = (local / sc.gx) % sc.gy;
+                    const int z = local / (sc.gx * sc.gy);
+                    if (x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1)
+                    {
+                        closest = d2;
+                        bestSlot = slot;
+                    }
+                }
+            }
+        }
+        else
+        {
+            walkSlots(c, sc, st.lostEnd, q.x, q.y, q.z, hh, [&](uint32_t slot) {
+                const float d2 = visit(slot);
+                if (d2 >= 0.0F)
                 {
                     closest = d2;
                     bestSlot = slot;
                 }
-            }
-        });
+            });
+        }
         c.cand[st.denseOff + id] = bestSlot == UINT32_MAX ? kNoCandidate : c.sortedVals[bestSlot] - sc.pOff;
     }
 }
