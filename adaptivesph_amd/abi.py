@@ -230,6 +230,7 @@ def _declare(lib: C.CDLL) -> None:
         "asph_default_parameters": (C.c_int, [P(Parameters)]),
         "asph_default_refinement_parameters": (C.c_int, [P(RefinementParameters)]),
         "asph_boundary_generate": (C.c_int, [P(Domain), f32, f32, f32, vp, P(u32)]),
+        "asph_boundary_generate_device": (C.c_int, [P(Domain), f32, f32, f32, vp, P(u32)]),
         "asph_create": (
             C.c_int,
             [P(Parameters), vp, u32, vp, u32, P(RefinementParameters), u32, u32, P(vp)],

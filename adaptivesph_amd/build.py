@@ -19,7 +19,7 @@ LIB = PKG / "lib" / "libasph.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("ASPH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["sort.hip", "sph.hip", "engine.cpp", "boundary.cpp", "abi.cpp"]
+SOURCES = ["sort.hip", "sph.hip", "walls.hip", "engine.cpp", "boundary.cpp", "abi.cpp"]
 
 FLAGS = [
     "-O3",

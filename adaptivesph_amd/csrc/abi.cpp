@@ -14,11 +14,7 @@
 #include "../../include/asph_debug.h"
 #include "engine.hpp"
 
-namespace asph
-{
-int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
-                     uint32_t* count);
-}
+#include "walls.hpp"
 
 struct asph_simulation
 {
@@ -142,7 +138,27 @@ int asph_boundary_generate(const asph_domain* domain, float particle_spacing, fl
         {
             throw asph::Error(ASPH_ERR_INVALID, "spacing and smoothing radius must be positive");
         }
-        const int rc = asph::boundaryGenerate(*domain, particle_spacing, rest_density, smoothing_radius, out, count);
+        const int rc =
+            asph::boundaryGenerate(*domain, particle_spacing, rest_density, smoothing_radius, out, count, false);
+        if (rc != ASPH_OK)
+        {
+            throw asph::Error(rc, "output array too small");
+        }
+    });
+}
+
+int asph_boundary_generate_device(const asph_domain* domain, float particle_spacing, float rest_density,
+                                  float smoothing_radius, asph_boundary_particle* out, uint32_t* count)
+{
+    return guard([&] {
+        need(domain, "domain");
+        need(count, "count");
+        if (!(particle_spacing > 0.0F) || !(smoothing_radius > 0.0F))
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "spacing and smoothing radius must be positive");
+        }
+        const int rc =
+            asph::boundaryGenerate(*domain, particle_spacing, rest_density, smoothing_radius, out, count, true);
         if (rc != ASPH_OK)
         {
             throw asph::Error(rc, "output array too small");

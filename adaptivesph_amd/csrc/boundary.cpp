@@ -13,17 +13,13 @@
 #include <vector>
 
 #include "../../include/asph.h"
+#include "walls.hpp"
 
 namespace asph
 {
 
 namespace
 {
-
-struct P4
-{
-    float x, y, z, w;
-};
 
 float wendlandHost(float d, float h)
 {
@@ -93,6 +89,31 @@ std::vector<P4> wallLattice(const asph_domain& d, float spacing)
     return out;
 }
 
+WallBuckets wallBuckets(const std::vector<P4>& pts, float h)
+{
+    WallBuckets wb;
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int a = 0; a < 3; ++a)
+    {
+        wb.lo[a] = DBL_MAX;
+    }
+    for (const auto& q : pts)
+    {
+        const double v[3] = {q.x, q.y, q.z};
+        for (int a = 0; a < 3; ++a)
+        {
+            wb.lo[a] = std::min(wb.lo[a], v[a]);
+            hi[a] = std::max(hi[a], v[a]);
+        }
+    }
+    wb.w = static_cast<double>(2.0F * h);
+    for (int a = 0; a < 3; ++a)
+    {
+        wb.dims[a] = std::max(1, static_cast<int>(std::floor((hi[a] - wb.lo[a]) / wb.w)) + 1);
+    }
+    return wb;
+}
+
 void wallPsi(const std::vector<P4>& pts, float h, float rho0, std::vector<float>& psi)
 {
     const size_t n = pts.size();
@@ -103,23 +124,10 @@ void wallPsi(const std::vector<P4>& pts, float h, float rho0, std::vector<float>
     }
     const float R = 2.0F * h;
     const float R2 = R * R;
-    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
-    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
-    for (const auto& q : pts)
-    {
-        const double v[3] = {q.x, q.y, q.z};
-        for (int a = 0; a < 3; ++a)
-        {
-            lo[a] = std::min(lo[a], v[a]);
-            hi[a] = std::max(hi[a], v[a]);
-        }
-    }
-    const double w = static_cast<double>(R);
-    int dims[3];
-    for (int a = 0; a < 3; ++a)
-    {
-        dims[a] = std::max(1, static_cast<int>(std::floor((hi[a] - lo[a]) / w)) + 1);
-    }
+    const WallBuckets wb = wallBuckets(pts, h);
+    const double* lo = wb.lo;
+    const int* dims = wb.dims;
+    const double w = wb.w;
     auto bucket = [&](const P4& q, int a) {
         const double v = a == 0 ? q.x : (a == 1 ? q.y : q.z);
         const int b = static_cast<int>(std::floor((v - lo[a]) / w));
@@ -209,7 +217,7 @@ void wallPsi(const std::vector<P4>& pts, float h, float rho0, std::vector<float>
 }
 
 int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
-                     uint32_t* count)
+                     uint32_t* count, bool onDevice)
 {
     const std::vector<P4> pts = wallLattice(d, spacing);
     if (out == nullptr)
@@ -222,7 +230,14 @@ int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, a
         return ASPH_ERR_CAPACITY;
     }
     std::vector<float> psi;
-    wallPsi(pts, h, rho0, psi);
+    if (onDevice)
+    {
+        wallPsiDevice(pts, h, rho0, psi);
+    }
+    else
+    {
+        wallPsi(pts, h, rho0, psi);
+    }
     for (size_t i = 0; i < pts.size(); ++i)
     {
         out[i].position = {pts[i].x, pts[i].y, pts[i].z, pts[i].w};

@@ -183,6 +183,14 @@ int asph_boundary_generate(const asph_domain* domain, float particle_spacing,
                            float rest_density, float smoothing_radius,
                            asph_boundary_particle* out, uint32_t* count);
 
+/* The same walls with psi computed on the current GPU
+   (StaticBoundaryDomain.cu:149-174 replaced by a bucket-grid radius sum
+   summed in the host generator's order): identical records, for domains
+   that are regenerated while the simulation runs. Synchronous. */
+int asph_boundary_generate_device(const asph_domain* domain, float particle_spacing,
+                                  float rest_density, float smoothing_radius,
+                                  asph_boundary_particle* out, uint32_t* count);
+
 /* createSimulation (Simulation.cuh:138-145, SimulationFactory.cu:14-40).
    The adaptive variant is chosen when refinement != NULL and
    refinement->enabled; it then uses refinement->max_particle_count as the
