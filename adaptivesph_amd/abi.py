@@ -238,6 +238,7 @@ def _declare(lib: C.CDLL) -> None:
         "asph_destroy": (C.c_int, [vp]),
         "asph_update": (C.c_int, [vp, f32]),
         "asph_update_domain": (C.c_int, [vp, P(Domain), vp, u32]),
+        "asph_update_domain_walls": (C.c_int, [vp, P(Domain), f32]),
         "asph_fluid_count": (C.c_int, [vp, P(u32)]),
         "asph_boundary_count": (C.c_int, [vp, P(u32)]),
         "asph_average_neighbor_count": (C.c_int, [vp, P(f32)]),

@@ -241,6 +241,14 @@ int asph_update_domain(asph_simulation* sim, const asph_domain* domain, const as
     });
 }
 
+int asph_update_domain_walls(asph_simulation* sim, const asph_domain* domain, float particle_spacing)
+{
+    return guard([&] {
+        need(domain, "domain");
+        engineOf(sim).updateDomainWalls(sim->scene, *domain, particle_spacing);
+    });
+}
+
 int asph_fluid_count(asph_simulation* sim, uint32_t* count)
 {
     return guard([&] {

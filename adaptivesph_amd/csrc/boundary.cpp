@@ -9,7 +9,9 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
+#include <stdexcept>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "../../include/asph.h"
@@ -87,6 +89,92 @@ std::vector<P4> wallLattice(const asph_domain& d, float spacing)
         }
     }
     return out;
+}
+
+WallFaces wallFaces(const asph_domain& d, float spacing, float h)
+{
+    // the loops of wallLattice: X, Y, Z from the min corner (the faces that
+    // start one step in begin at the sequence's second value: the same FP32
+    // additions)
+    const float off = 0.05F;
+    const float mn[3] = {d.min.x - off, d.min.y - off, d.min.z - off};
+    const float mx[3] = {d.max.x + off, d.max.y + off, d.max.z + off};
+    WallFaces wf;
+    uint32_t seqOff[3];
+    uint32_t seqLen[3];
+    for (int a = 0; a < 3; ++a)
+    {
+        seqOff[a] = static_cast<uint32_t>(wf.coords.size());
+        for (float v = mn[a]; v < mx[a]; v += spacing)
+        {
+            wf.coords.push_back(v);
+        }
+        seqLen[a] = static_cast<uint32_t>(wf.coords.size()) - seqOff[a];
+    }
+    auto full = [&](int a) { return std::pair<uint32_t, uint32_t>(seqOff[a], seqLen[a]); };
+    auto tail = [&](int a) {
+        return seqLen[a] == 0U ? std::pair<uint32_t, uint32_t>(seqOff[a], 0U)
+                               : std::pair<uint32_t, uint32_t>(seqOff[a] + 1U, seqLen[a] - 1U);
+    };
+    struct Spec
+    {
+        std::pair<uint32_t, uint32_t> A, B;
+        int axisA, axisB, axisFixed;
+        float fixed;
+    };
+    const Spec spec[6] = {
+        {full(0), full(2), 0, 2, 1, mn[1]}, {full(0), full(2), 0, 2, 1, mx[1]},
+        {tail(1), full(2), 1, 2, 0, mn[0]}, {tail(1), full(2), 1, 2, 0, mx[0]},
+        {tail(0), tail(1), 0, 1, 2, mn[2]}, {tail(0), tail(1), 0, 1, 2, mx[2]},
+    };
+    uint64_t start = 0;
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int f = 0; f < 6; ++f)
+    {
+        const Spec& sp = spec[f];
+        WallFace& w = wf.face[f];
+        w.offA = sp.A.first;
+        w.nA = sp.A.second;
+        w.offB = sp.B.first;
+        w.nB = sp.B.second;
+        w.axisA = sp.axisA;
+        w.axisB = sp.axisB;
+        w.axisFixed = sp.axisFixed;
+        w.fixed = sp.fixed;
+        w.start = static_cast<uint32_t>(start);
+        const uint64_t n = static_cast<uint64_t>(w.nA) * w.nB;
+        start += n;
+        if (n == 0)
+        {
+            continue;
+        }
+        // the face's extent (the bucket grid spans the particles themselves)
+        auto span = [&](uint32_t o, uint32_t len, int axis) {
+            for (uint32_t k = o; k < o + len; ++k)
+            {
+                lo[axis] = std::min(lo[axis], static_cast<double>(wf.coords[k]));
+                hi[axis] = std::max(hi[axis], static_cast<double>(wf.coords[k]));
+            }
+        };
+        span(w.offA, w.nA, w.axisA);
+        span(w.offB, w.nB, w.axisB);
+        lo[w.axisFixed] = std::min(lo[w.axisFixed], static_cast<double>(w.fixed));
+        hi[w.axisFixed] = std::max(hi[w.axisFixed], static_cast<double>(w.fixed));
+    }
+    if (start >= (1ULL << 31))
+    {
+        throw std::runtime_error("too many wall particles");
+    }
+    wf.count = static_cast<uint32_t>(start);
+    wf.buckets.w = static_cast<double>(2.0F * h);
+    for (int a = 0; a < 3; ++a)
+    {
+        wf.buckets.lo[a] = wf.count == 0 ? 0.0 : lo[a];
+        wf.buckets.dims[a] =
+            wf.count == 0 ? 1 : std::max(1, static_cast<int>(std::floor((hi[a] - lo[a]) / wf.buckets.w)) + 1);
+    }
+    return wf;
 }
 
 WallBuckets wallBuckets(const std::vector<P4>& pts, float h)

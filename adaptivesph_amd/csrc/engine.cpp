@@ -1,6 +1,8 @@
 // Host side of the HIP engine (see engine.hpp).
 #include "engine.hpp"
 
+#include "walls.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -923,6 +925,48 @@ void Engine::setDt(uint32_t id, float dt)
 
 void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_boundary_particle* boundary, uint32_t nb)
 {
+    prepareDomain(id, dom, nb);
+    uploadBoundary(id, boundary, nb);
+    uploadScene(id);
+    uploadOrder();
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacing)
+{
+    if (!(spacing > 0.0F))
+    {
+        throw Error(ASPH_ERR_INVALID, "particle spacing must be positive");
+    }
+    HostScene& hs0 = scenes_.at(id);
+    const WallFaces wf = wallFaces(dom, spacing, hs0.dev.h0);
+    prepareDomain(id, dom, wf.count);
+    HostScene& hs = scenes_.at(id);
+    if (wf.count == 0)
+    {
+        uploadBoundary(id, nullptr, 0);
+    }
+    else
+    {
+        SceneDev& d = hs.dev;
+        d.bCount = wf.count;
+        const WallTargets out {ctx_.bs0,  ctx_.bvisc, ctx_.bcellOff, ctx_.bbox0,
+                               ctx_.bbox1, ctx_.bgbox0, ctx_.bgbox1, ctx_.bnear2};
+        uint32_t lost = d.bOff;
+        bool inside = false;
+        wallsOnDevice(wf, d, out, &lost, &inside);
+        d.bLostEnd = lost;
+        d.flags = inside ? (d.flags | kSceneBoundaryInside) : (d.flags & ~kSceneBoundaryInside);
+        hs.boundaryKeysSorted.clear();
+        hs.boundarySorted.clear();
+    }
+    uploadScene(id);
+    uploadOrder();
+    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+}
+
+void Engine::prepareDomain(uint32_t id, const asph_domain& dom, uint32_t nb)
+{
     HostScene& hs = scenes_.at(id);
     if (!(dom.max.x > dom.min.x) || !(dom.max.y > dom.min.y) || !(dom.max.z > dom.min.z))
     {
@@ -971,10 +1015,6 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     d.bOff = static_cast<uint32_t>(bOff);
     hs.cellRegion = static_cast<uint32_t>(cells + 1);
     hipCheck(hipMemset(ctx_.cellOff + cBase, 0, sizeof(uint32_t) * (cells + 1)), "hipMemset");
-    uploadBoundary(id, boundary, nb);
-    uploadScene(id);
-    uploadOrder();
-    hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
 template <class T>

@@ -104,6 +104,9 @@ public:
     bool sceneActive(uint32_t id) const;
     void setDt(uint32_t id, float dt);
     void updateDomain(uint32_t id, const asph_domain& d, const asph_boundary_particle* boundary, uint32_t nb);
+    // updateDomain with the walls of StaticBoundaryDomain::generate(d,
+    // spacing, rho0, h0) built on the device (no particle data through the host)
+    void updateDomainWalls(uint32_t id, const asph_domain& d, float spacing);
     void enableRefinement(uint32_t id);
     void setFrame(uint32_t id, uint32_t frame);
 
@@ -165,6 +168,8 @@ private:
     void uploadOrder();
     void updateExtents();
     void uploadBoundary(uint32_t id, const asph_boundary_particle* b, uint32_t nb);
+    // the new domain's grid and pool regions for nb boundary particles
+    void prepareDomain(uint32_t id, const asph_domain& d, uint32_t nb);
     void nearBounds(const SceneDev& d, const std::vector<F4>& glo, const std::vector<F4>& ghi);
     void enqueueStep(bool refine, std::vector<hipEvent_t>* marks);
     bool anyRefineNext() const;

@@ -212,6 +212,224 @@ __global__ __launch_bounds__(kPsiBlock) void kWallPsi(const F4* pos, uint32_t n,
     }
 }
 
+// ----------------------------------------------- walls of a new domain --
+struct FaceSet
+{
+    WallFace f[6];
+};
+
+// StaticBoundaryDomain::generate's lattice (wallLattice) from its coordinate
+// sequences
+__global__ __launch_bounds__(256) void kWallLattice(FaceSet fs, const float* coords, uint32_t n, F4* pos)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        int f = 0;
+        while (f < 5 && !(i >= fs.f[f].start && i - fs.f[f].start < fs.f[f].nA * fs.f[f].nB))
+        {
+            ++f;
+        }
+        const WallFace& w = fs.f[f];
+        const uint32_t l = i - w.start;
+        const uint32_t a = l / w.nB;
+        const uint32_t b = l - a * w.nB;
+        float v[3];
+        v[w.axisA] = coords[w.offA + a];
+        v[w.axisB] = coords[w.offB + b];
+        v[w.axisFixed] = w.fixed;
+        pos[i] = F4 {v[0], v[1], v[2], 0.0F};
+    }
+}
+
+// NeighborGrid::updateBoundary (NeighborGrid.cu:42-49): the boundary
+// particles' cell keys
+__global__ __launch_bounds__(256) void kBoundaryKeys(const F4* pos, uint32_t n, SceneDev d, uint32_t* keys,
+                                                     uint32_t* vals)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const F4 p = pos[i];
+        keys[i] = cellKey(d, p.x, p.y, p.z);
+        vals[i] = i;
+    }
+}
+
+// sorted boundary records (position, psi) and friction coefficients; the
+// chunk boxes cleared to empty
+__global__ __launch_bounds__(256) void kBoundaryScatter(const F4* pos, const float* psi, const uint32_t* sortedVals,
+                                                        uint32_t n, float friction, F4* bs0, float* bvisc, F4* bg0,
+                                                        F4* bg1)
+{
+    const float inf = __builtin_huge_valf();
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+    {
+        const uint32_t i = sortedVals[k];
+        const F4 p = pos[i];
+        bs0[k] = F4 {p.x, p.y, p.z, psi[i]};
+        bvisc[k] = friction;
+        bg0[k] = F4 {inf, inf, inf, 0.0F};
+        bg1[k] = F4 {-inf, -inf, -inf, 0.0F};
+    }
+}
+
+__device__ __forceinline__ uint32_t lowerBoundKeys(const uint32_t* keys, uint32_t n, uint32_t key)
+{
+    uint32_t lo = 0;
+    uint32_t hi = n;
+    while (lo < hi)
+    {
+        const uint32_t mid = (lo + hi) >> 1U;
+        if (keys[mid] < key)
+        {
+            lo = mid + 1U;
+        }
+        else
+        {
+            hi = mid;
+        }
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void kNearInit(unsigned long long* near, uint32_t cells, double cap2)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x)
+    {
+        near[c] = static_cast<unsigned long long>(__double_as_longlong(cap2));
+    }
+}
+
+// Per cell (and the end entry): the CSR offset of its boundary slots, the box
+// of all its slots, the boxes of its chunks of kBoundaryGroup slots (stored at
+// each chunk's first slot) and each chunk's lower bound of the squared
+// distance to the cells around it (min over chunks; as the host ingestion).
+__global__ __launch_bounds__(256) void kBoundaryCells(const uint32_t* sortedKeys, uint32_t n, SceneDev d,
+                                                      const F4* bs0, uint32_t* bcellOff, F4* bb0, F4* bb1, F4* bg0,
+                                                      F4* bg1, unsigned long long* near, double cap)
+{
+    const float inf = __builtin_huge_valf();
+    const uint32_t cells = d.nCells;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c <= cells; c += gridDim.x * blockDim.x)
+    {
+        const uint32_t first = lowerBoundKeys(sortedKeys, n, c);
+        bcellOff[c] = d.bOff + first;
+        F4 lo {inf, inf, inf, 0.0F};
+        F4 hi {-inf, -inf, -inf, 0.0F};
+        if (c < cells)
+        {
+            const uint32_t end = lowerBoundKeys(sortedKeys, n, c + 1U);
+            for (uint32_t k0 = first; k0 < end; k0 += kBoundaryGroup)
+            {
+                const uint32_t k1 = min(k0 + kBoundaryGroup, end);
+                F4 glo {inf, inf, inf, 0.0F};
+                F4 ghi {-inf, -inf, -inf, 0.0F};
+                for (uint32_t k = k0; k < k1; ++k)
+                {
+                    const F4 p = bs0[k];
+                    glo.x = (p.x < glo.x) ? p.x : glo.x;
+                    glo.y = (p.y < glo.y) ? p.y : glo.y;
+                    glo.z = (p.z < glo.z) ? p.z : glo.z;
+                    ghi.x = (ghi.x < p.x) ? p.x : ghi.x;
+                    ghi.y = (ghi.y < p.y) ? p.y : ghi.y;
+                    ghi.z = (ghi.z < p.z) ? p.z : ghi.z;
+                }
+                bg0[k0] = glo;
+                bg1[k0] = ghi;
+                lo.x = (glo.x < lo.x) ? glo.x : lo.x;
+                lo.y = (glo.y < lo.y) ? glo.y : lo.y;
+                lo.z = (glo.z < lo.z) ? glo.z : lo.z;
+                hi.x = (hi.x < ghi.x) ? ghi.x : hi.x;
+                hi.y = (hi.y < ghi.y) ? ghi.y : hi.y;
+                hi.z = (hi.z < ghi.z) ? ghi.z : hi.z;
+                // cells within `cap` of the chunk box, as Engine::nearBounds
+                const double cl3[3] = {glo.x, glo.y, glo.z};
+                const double ch3[3] = {ghi.x, ghi.y, ghi.z};
+                const double cs[3] = {d.cs[0], d.cs[1], d.cs[2]};
+                const double mn[3] = {d.dmin[0], d.dmin[1], d.dmin[2]};
+                const int gdim[3] = {d.gx, d.gy, d.gz};
+                int i0[3];
+                int i1[3];
+                for (int a = 0; a < 3; ++a)
+                {
+                    i0[a] = max(static_cast<int>(floor((cl3[a] - cap - mn[a]) / cs[a])) - 1, 0);
+                    i1[a] = min(static_cast<int>(floor((ch3[a] + cap - mn[a]) / cs[a])) + 1, gdim[a] - 1);
+                }
+                for (int z = i0[2]; z <= i1[2]; ++z)
+                {
+                    for (int y = i0[1]; y <= i1[1]; ++y)
+                    {
+                        for (int x = i0[0]; x <= i1[0]; ++x)
+                        {
+                            const int idx[3] = {x, y, z};
+                            double d2 = 0.0;
+                            for (int a = 0; a < 3; ++a)
+                            {
+                                const double m = 1e-3 * cs[a];
+                                const double cl = mn[a] + idx[a] * cs[a] - m;
+                                const double ch = mn[a] + (idx[a] + 1) * cs[a] + m;
+                                const double gap = fmax(0.0, fmax(cl3[a] - ch, cl - ch3[a]));
+                                d2 += gap * gap;
+                            }
+                            const size_t cell = static_cast<size_t>(x) + static_cast<size_t>(y) * gdim[0] +
+                                                static_cast<size_t>(z) * static_cast<size_t>(gdim[0]) *
+                                                    static_cast<size_t>(gdim[1]);
+                            atomicMin(near + cell, static_cast<unsigned long long>(__double_as_longlong(d2)));
+                        }
+                    }
+                }
+            }
+        }
+        bb0[c] = lo;
+        bb1[c] = hi;
+    }
+}
+
+__global__ __launch_bounds__(256) void kNearFinal(const unsigned long long* near, uint32_t cells, float* bnear2)
+{
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x)
+    {
+        const double v = __longlong_as_double(static_cast<long long>(near[c]));
+        const float f = static_cast<float>(v * (1.0 - 1e-5));
+        bnear2[c] = f > 0.0F ? nextafterf(f, 0.0F) : 0.0F;
+    }
+}
+
+// meta[0]: end of the boundary slots the slot-0 omission hides
+// (NeighborKernels.cu:17-44); meta[1]: 1 if a boundary particle lies in an
+// interior cell
+__global__ __launch_bounds__(256) void kBoundaryMeta(const uint32_t* sortedKeys, uint32_t n, SceneDev d,
+                                                     uint32_t* meta)
+{
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t0 == 0U)
+    {
+        uint32_t lost = d.bOff;
+        if ((d.flags & kSceneFixCellStart) == 0U && n > 0U)
+        {
+            if (n == 1U)
+            {
+                lost = d.bOff + 1U;
+            }
+            else if (sortedKeys[0] != sortedKeys[1])
+            {
+                lost = d.bOff + lowerBoundKeys(sortedKeys, n, sortedKeys[1] + 1U);
+            }
+        }
+        meta[0] = lost;
+    }
+    for (uint32_t k = t0; k < n; k += gridDim.x * blockDim.x)
+    {
+        const int key = static_cast<int>(sortedKeys[k]);
+        const int cx = key % d.gx;
+        const int cy = (key / d.gx) % d.gy;
+        const int cz = key / (d.gx * d.gy);
+        if (cx != 0 && cy != 0 && cz != 0 && cx != d.gx - 1 && cy != d.gy - 1 && cz != d.gz - 1)
+        {
+            atomicOr(meta + 1, 1U);
+        }
+    }
+}
+
 template <typename T>
 struct DevBuf
 {
@@ -227,20 +445,13 @@ struct DevBuf
 
 }  // namespace
 
-void wallPsiDevice(const std::vector<P4>& pts, float h, float rho0, std::vector<float>& psi)
+void wallPsiOnDevice(const F4* pos, uint32_t n, const WallBuckets& wb, float h, float rho0, float* psi,
+                     hipStream_t s)
 {
-    const size_t n64 = pts.size();
-    psi.assign(n64, 0.0F);
-    if (n64 == 0)
+    if (n == 0)
     {
         return;
     }
-    if (n64 >= (1ULL << 31))
-    {
-        throw std::runtime_error("too many wall particles");
-    }
-    const uint32_t n = static_cast<uint32_t>(n64);
-    const WallBuckets wb = wallBuckets(pts, h);
     BucketGrid g;
     for (int a = 0; a < 3; ++a)
     {
@@ -261,16 +472,12 @@ void wallPsiDevice(const std::vector<P4>& pts, float h, float rho0, std::vector<
         ++bits;
     }
     const int radixBits = radixDigitBits(0, bits);
-    DevBuf<F4> pos(n);
     DevBuf<uint32_t> keysA(n), valsA(n), keysB(n), valsB(n);
     DevBuf<uint32_t> start(static_cast<size_t>(buckets) + 1U);
     DevBuf<uint32_t> hist(radixHistWords(n, radixBits));
     DevBuf<uint32_t> digitTotals(static_cast<size_t>(1U) << radixBits);
-    DevBuf<float> out(n);
-    hipStream_t s = nullptr;
-    hipCheck(hipMemcpy(pos.p, pts.data(), sizeof(F4) * n, hipMemcpyHostToDevice), "upload");
     const uint32_t blocks = std::min<uint32_t>((n + 255U) / 256U, 4096U);
-    hipLaunchKernelGGL(kWallKeys, dim3(blocks), dim3(256), 0, s, pos.p, n, g, keysA.p, valsA.p);
+    hipLaunchKernelGGL(kWallKeys, dim3(blocks), dim3(256), 0, s, pos, n, g, keysA.p, valsA.p);
     const int inB = radixSortU32(keysA.p, valsA.p, keysB.p, valsB.p, nullptr, n, bits, radixBits, hist.p,
                                  digitTotals.p, s);
     const uint32_t* sk = inB != 0 ? keysB.p : keysA.p;
@@ -278,9 +485,87 @@ void wallPsiDevice(const std::vector<P4>& pts, float h, float rho0, std::vector<
     const uint32_t bblocks = std::min<uint32_t>((buckets + 256U) / 256U, 4096U);
     hipLaunchKernelGGL(kWallBucketStarts, dim3(bblocks), dim3(256), 0, s, sk, n, buckets, start.p);
     const uint32_t pblocks = std::min<uint32_t>((n + kPsiBlock - 1U) / kPsiBlock, 16384U);
-    hipLaunchKernelGGL(kWallPsi, dim3(pblocks), dim3(kPsiBlock), 0, s, pos.p, n, g, sv, start.p, h, rho0, out.p);
+    hipLaunchKernelGGL(kWallPsi, dim3(pblocks), dim3(kPsiBlock), 0, s, pos, n, g, sv, start.p, h, rho0, psi);
     hipCheck(hipGetLastError(), "wall psi launch");
+    // the temporaries are freed when this returns: finish with them first
+    hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+}
+
+void wallPsiDevice(const std::vector<P4>& pts, float h, float rho0, std::vector<float>& psi)
+{
+    const size_t n64 = pts.size();
+    psi.assign(n64, 0.0F);
+    if (n64 == 0)
+    {
+        return;
+    }
+    if (n64 >= (1ULL << 31))
+    {
+        throw std::runtime_error("too many wall particles");
+    }
+    const uint32_t n = static_cast<uint32_t>(n64);
+    DevBuf<F4> pos(n);
+    DevBuf<float> out(n);
+    hipCheck(hipMemcpy(pos.p, pts.data(), sizeof(F4) * n, hipMemcpyHostToDevice), "upload");
+    wallPsiOnDevice(pos.p, n, wallBuckets(pts, h), h, rho0, out.p, nullptr);
     hipCheck(hipMemcpy(psi.data(), out.p, sizeof(float) * n, hipMemcpyDeviceToHost), "download");
+}
+
+void wallsOnDevice(const WallFaces& wf, const SceneDev& d, const WallTargets& out, uint32_t* lostEnd, bool* inside)
+{
+    const uint32_t n = wf.count;
+    const uint32_t cells = d.nCells;
+    hipStream_t s = nullptr;
+    DevBuf<float> coords(wf.coords.size());
+    DevBuf<F4> pos(n);
+    DevBuf<float> psi(n);
+    DevBuf<uint32_t> keysA(n), valsA(n), keysB(n), valsB(n);
+    DevBuf<unsigned long long> near(cells);
+    DevBuf<uint32_t> meta(2);
+    if (!wf.coords.empty())
+    {
+        hipCheck(hipMemcpy(coords.p, wf.coords.data(), sizeof(float) * wf.coords.size(), hipMemcpyHostToDevice),
+                 "upload");
+    }
+    FaceSet fs;
+    for (int f = 0; f < 6; ++f)
+    {
+        fs.f[f] = wf.face[f];
+    }
+    const uint32_t blocks = std::max<uint32_t>(std::min<uint32_t>((n + 255U) / 256U, 4096U), 1U);
+    hipLaunchKernelGGL(kWallLattice, dim3(blocks), dim3(256), 0, s, fs, coords.p, n, pos.p);
+    wallPsiOnDevice(pos.p, n, wf.buckets, d.h0, d.rho0, psi.p, s);
+    hipLaunchKernelGGL(kBoundaryKeys, dim3(blocks), dim3(256), 0, s, pos.p, n, d, keysA.p, valsA.p);
+    int bits = 1;
+    while (bits < 32 && (1ULL << bits) <= static_cast<uint64_t>(cells))
+    {
+        ++bits;
+    }
+    const int radixBits = radixDigitBits(0, bits);
+    DevBuf<uint32_t> hist(radixHistWords(n, radixBits));
+    DevBuf<uint32_t> digitTotals(static_cast<size_t>(1U) << radixBits);
+    const int inB = radixSortU32(keysA.p, valsA.p, keysB.p, valsB.p, nullptr, n, bits, radixBits, hist.p,
+                                 digitTotals.p, s);
+    const uint32_t* sk = inB != 0 ? keysB.p : keysA.p;
+    const uint32_t* sv = inB != 0 ? valsB.p : valsA.p;
+    hipLaunchKernelGGL(kBoundaryScatter, dim3(blocks), dim3(256), 0, s, pos.p, psi.p, sv, n, d.friction,
+                       out.bs0 + d.bOff, out.bvisc + d.bOff, out.bg0 + d.bOff, out.bg1 + d.bOff);
+    const double cs = std::max(static_cast<double>(d.cs[0]),
+                               std::max(static_cast<double>(d.cs[1]), static_cast<double>(d.cs[2])));
+    const double cap = 2.0 * cs;
+    const uint32_t cblocks = std::max<uint32_t>(std::min<uint32_t>((cells + 256U) / 256U, 4096U), 1U);
+    hipLaunchKernelGGL(kNearInit, dim3(cblocks), dim3(256), 0, s, near.p, cells, cap * cap);
+    hipLaunchKernelGGL(kBoundaryCells, dim3(cblocks), dim3(256), 0, s, sk, n, d, out.bs0 + d.bOff,
+                       out.bcellOff + d.cellBase, out.bb0 + d.cellBase, out.bb1 + d.cellBase, out.bg0 + d.bOff,
+                       out.bg1 + d.bOff, near.p, cap);
+    hipLaunchKernelGGL(kNearFinal, dim3(cblocks), dim3(256), 0, s, near.p, cells, out.bnear2 + d.cellBase);
+    hipCheck(hipMemset(meta.p, 0, sizeof(uint32_t) * 2), "hipMemset");
+    hipLaunchKernelGGL(kBoundaryMeta, dim3(blocks), dim3(256), 0, s, sk, n, d, meta.p);
+    hipCheck(hipGetLastError(), "wall binning launch");
+    uint32_t m[2] = {0, 0};
+    hipCheck(hipMemcpy(m, meta.p, sizeof(m), hipMemcpyDeviceToHost), "download");
+    *lostEnd = m[0];
+    *inside = m[1] != 0U;
 }
 
 }  // namespace asph

@@ -3,6 +3,8 @@
 // current GPU (walls.hip), both with the same buckets and summation order.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 #include <vector>
 
@@ -35,6 +37,57 @@ WallBuckets wallBuckets(const std::vector<P4>& pts, float h);
 // 2h support (strict d^2 < (2h)^2, i itself included), summed in ascending j.
 void wallPsi(const std::vector<P4>& pts, float h, float rho0, std::vector<float>& psi);
 void wallPsiDevice(const std::vector<P4>& pts, float h, float rho0, std::vector<float>& psi);
+
+// The lattice as six product faces over the FP32 coordinate sequences of
+// wallLattice (x, y and z from the min corner 0.05 m outside the domain in
+// steps of `spacing`): face f holds, in order, for each a of its outer
+// sequence and each b of its inner one, the particle with coordinate a on
+// axisA, b on axisB and `fixed` on the remaining axis. coords = X ++ Y ++ Z.
+struct WallFace
+{
+    uint32_t offA, nA, offB, nB;  // ranges of coords
+    int axisA, axisB, axisFixed;
+    float fixed;
+    uint32_t start;  // first particle of the face
+};
+
+struct WallFaces
+{
+    std::vector<float> coords;
+    WallFace face[6];
+    uint32_t count;
+    WallBuckets buckets;  // wallBuckets of the lattice, for psi with smoothing radius h
+};
+
+WallFaces wallFaces(const asph_domain& d, float spacing, float h);
+
+// psi of n device-resident wall positions (w ignored) into psi[n] (device);
+// synchronous on stream s.
+struct F4;
+void wallPsiOnDevice(const F4* pos, uint32_t n, const WallBuckets& wb, float h, float rho0, float* psi,
+                     hipStream_t s);
+
+// Device destinations of a scene's boundary particles (the engine's pools,
+// unoffset: the scene's bOff and cellBase are applied by wallsOnDevice).
+struct SceneDev;
+struct WallTargets
+{
+    F4* bs0;
+    float* bvisc;
+    uint32_t* bcellOff;
+    F4* bb0;
+    F4* bb1;
+    F4* bg0;
+    F4* bg1;
+    float* bnear2;
+};
+
+// The walls of the domain of scene description d (whose grid, pool offsets,
+// friction, h0 and rest density are already those of the new domain), built
+// on the current GPU straight into the engine's boundary arrays, binned as
+// the host ingestion bins them; returns the end of the slot-0 omission and
+// whether a wall particle lies in an interior cell. wf.count > 0. Synchronous.
+void wallsOnDevice(const WallFaces& wf, const SceneDev& d, const WallTargets& out, uint32_t* lostEnd, bool* inside);
 
 // asph_boundary_generate (onDevice false) and asph_boundary_generate_device.
 int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,

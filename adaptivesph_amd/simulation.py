@@ -71,6 +71,12 @@ class Simulation:
         self._check(self._lib.asph_update_domain(self._h, C.byref(domain), b.ctypes.data if len(b) else None,
                                                  len(b)), "updateDomain")
 
+    def update_domain_walls(self, domain: abi.Domain, spacing: float) -> None:
+        """updateDomain with StaticBoundaryDomain::generate's walls for `domain`
+        (spacing, the rest density and h0) built on the GPU."""
+        self._check(self._lib.asph_update_domain_walls(self._h, C.byref(domain), float(spacing)),
+                    "updateDomain (device walls)")
+
     def get_fluid_particles_count(self) -> int:
         n = C.c_uint32()
         self._check(self._lib.asph_fluid_count(self._h, C.byref(n)), "getFluidParticlesCount")

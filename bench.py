@@ -16,6 +16,9 @@ Other workloads (--workload):
             scenes stepped one scene after another
   serve     Poisson request stream admitted against a 200 GB device budget
             (continuous batching), 10k-500k particles, 100-2000 steps each
+  walls     updateDomain of the 1M dam break with its walls (851,860 wall
+            particles) regenerated for a moved box: walls, psi and binning on
+            the GPU, beside the host generator and host ingestion
   copy      1 GiB device copy (calibration of hardware byte counters)
 """
 from __future__ import annotations
@@ -288,6 +291,48 @@ def run_serve(args) -> dict:
     return out
 
 
+def run_walls(args) -> dict:
+    """Latency of updateDomain with regenerated walls (a moving tank): the
+    device path (asph_update_domain_walls) against the host path
+    (asph_boundary_generate + asph_update_domain), alternating between two
+    domains so every call rebuilds the walls."""
+    from adaptivesph_amd import abi, scene
+    from adaptivesph_amd.simulation import Simulation
+
+    s = scene.cfg2()
+    sim = Simulation.from_scene(s)
+    spacing = scene.boundary_spacing(s.params)
+    d0 = s.params.domain
+    domains = [abi.Domain(abi.Vec3(d0.min.x + 0.001 * k, d0.min.y, d0.min.z),
+                          abi.Vec3(d0.max.x + 0.001 * k, d0.max.y, d0.max.z), d0.friction) for k in (1, 2)]
+    for k in range(max(args.warmup, 1)):
+        sim.update_domain_walls(domains[k % 2], spacing)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        sim.update_domain_walls(domains[k % 2], spacing)
+    dev_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    nb = sim.get_boundary_particles_count()
+    log(f"walls: {nb} wall particles, device updateDomain {dev_ms:.2f} ms")
+    out = base_line(args, "updateDomain with regenerated walls (latency)", dev_ms, "ms", dev_ms,
+                    {"workload": "walls_dam_break_1M", "wall_particles": nb, "fluid_particles": s.count})
+    out["higher_is_better"] = False
+    if not args.no_cpu_baseline:
+        reps = 3
+        t1 = time.perf_counter()
+        for k in range(reps):
+            p = abi.Parameters.from_buffer_copy(s.params)
+            p.domain = domains[k % 2]
+            sim.update_domain(domains[k % 2], scene.generate_boundary(p))
+        host_ms = (time.perf_counter() - t1) / reps * 1e3
+        out["cpu_baseline"] = {"value": host_ms, "unit": "ms", "cores": min(16, os.cpu_count() or 1), "kind": "port",
+                               "sample": f"{reps} host updateDomain calls: asph_boundary_generate (psi on host "
+                                         "threads) + asph_update_domain (host binning, upload)"}
+    else:
+        out["cpu_baseline"] = None
+    sim.close()
+    return out
+
+
 def run_copy(args) -> dict:
     """A plain device-to-device copy of 1 GiB (torch's elementwise copy
     kernel): the known-size reference that scales hardware byte counters."""
@@ -356,6 +401,8 @@ def main() -> int:
         out = run_copy(args)
     elif args.workload == "batch":
         out = run_batch(args)
+    elif args.workload == "walls":
+        out = run_walls(args)
     elif args.workload == "serve":
         out = run_serve(args)
     else:

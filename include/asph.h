@@ -212,6 +212,15 @@ int asph_update(asph_simulation* sim, float dt);
 int asph_update_domain(asph_simulation* sim, const asph_domain* domain,
                        const asph_boundary_particle* boundary, uint32_t boundary_count);
 
+/* updateDomain(domain, StaticBoundaryDomain::generate(domain, particle_spacing,
+   rest_density, base_smoothing_radius)) as the application calls them
+   (App.cpp:143-148 and :377), with the walls, their psi and their binning
+   built on the GPU: no particle data passes through the host (for domains
+   that move while the simulation runs). Same state as asph_update_domain
+   with asph_boundary_generate's records. Synchronous. */
+int asph_update_domain_walls(asph_simulation* sim, const asph_domain* domain,
+                             float particle_spacing);
+
 /* getFluidParticlesCount / getBoundaryParticlesCount (Simulation.cuh:130-131). */
 int asph_fluid_count(asph_simulation* sim, uint32_t* count);
 int asph_boundary_count(asph_simulation* sim, uint32_t* count);
