@@ -382,7 +382,7 @@ def main() -> int:
     if args.list_capacity == 0:
         # the library's defaults: refined regions hold up to ~13x denser
         # daughters, and particles there have several hundred neighbours
-        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy") else 640
+        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy", "walls") else 640
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build steps scenes on one "
               "GPU and leaves multi-GPU sharding out", file=sys.stderr)
