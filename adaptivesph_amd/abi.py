@@ -67,6 +67,7 @@ class Parameters(C.Structure):
         ("slosh_amplitude", C.c_float),
         ("slosh_period", C.c_float),
         ("flags", C.c_uint32),
+        ("laminar_viscosity", C.c_float),
     ]
 
 
@@ -170,6 +171,7 @@ def default_parameters() -> Parameters:
     p.slosh_amplitude = 0.0
     p.slosh_period = 0.0
     p.flags = 0
+    p.laminar_viscosity = 0.0
     return p
 
 

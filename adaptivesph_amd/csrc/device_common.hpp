@@ -42,6 +42,7 @@ struct SceneDev
     float g[3];
     float c0, rho0, alpha, sigma, vmax, h0, r0, m0, gamma;
     float B;  // ((rho0 * c0) * c0) / gamma (Common.cuh:9)
+    float nuLam;  // kinematic viscosity of the Morris laminar term (0 = off)
     float sloshA, sloshT;
     float dt, halfDt;
     int gx, gy, gz;

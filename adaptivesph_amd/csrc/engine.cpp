@@ -169,6 +169,10 @@ void validateParams(const asph_parameters& p)
     {
         bad("rest_density must be positive");
     }
+    if (!(p.laminar_viscosity >= 0.0F) || !std::isfinite(p.laminar_viscosity))
+    {
+        bad("laminar_viscosity must be finite and non-negative");
+    }
     if (!(p.tait_gamma > 0.0F))
     {
         bad("tait_gamma must be positive");
@@ -527,6 +531,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     d.r0 = p.base_particle_radius;
     d.m0 = p.base_particle_mass;
     d.gamma = p.tait_gamma;
+    d.nuLam = p.laminar_viscosity;
     d.B = ((p.rest_density * p.speed_of_sound) * p.speed_of_sound) / p.tait_gamma;
     d.sloshA = p.slosh_amplitude;
     d.sloshT = p.slosh_period;

@@ -998,6 +998,7 @@ struct PairAcc
     float apx, apy, apz;
     float avx, avy, avz;
     float asx, asy, asz;
+    float amx, amy, amz;  // Morris laminar term (extension)
 };
 
 // One fluid pair of computePressureAccelerations (Algorithm.cu:72-105),
@@ -1007,10 +1008,10 @@ struct PairAcc
 // wdj = its mass, smoothing length and kWDCoef / h^4.
 // With kSharedH, yh = divRecip(hj) and hj lies in [2^-40, 2^40], so d / hj
 // is taken through the shared reciprocal (same bits as the division).
-template <bool kSharedH = false>
+template <bool kSharedH = false, bool kLam = false>
 __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, float hj, float wdj, const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
-                                          bool doST, PairAcc& A, float yh = 0.0F, bool use = true)
+                                          bool doST, float nuLam, PairAcc& A, float yh = 0.0F, bool use = true)
 {
     // use == false (the particle itself) computes and discards: the pair is
     // evaluated unconditionally so that the callers' gathers are not sunk
@@ -1071,6 +1072,16 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
             A.avz = A.avz - sv * dz;
         }
     }
+    if (kLam && use)
+    {
+        // Morris laminar term (extension): m_j nu (rho_i + rho_j) / (rho_i rho_j)
+        // * (r . grad W) / (r^2 + 0.01 h_i^2) * (v_i - v_j), r . grad W = d W'
+        const float k = ((mj * nuLam) * (rhoi + fb.w)) / (rhoi * fb.w);
+        const float sl = (k * (d * dw)) / (d2 + epsH2);
+        A.amx = A.amx + sl * (vi.x - fb.x);
+        A.amy = A.amy + sl * (vi.y - fb.y);
+        A.amz = A.amz + sl * (vi.z - fb.z);
+    }
     if (doST && use)
     {
         const float sw = mj * wendland(d, hj);
@@ -1084,14 +1095,23 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
 // walk again, in the same order. Out of line (rare), like densityWalk.
 __device__ __forceinline__ PairAcc forceWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
                                           const F4& q, const F4& vi, float rhoi, float prri, float nuNum, float epsH2,
-                                          bool doVisc, bool doST, uint32_t t)
+                                          bool doVisc, bool doST, float nuLam, uint32_t t)
 {
-    PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
+    PairAcc A {};
     walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
         if (j != t)
         {
             const F4 hm = c.shm[j];
-            forcePair(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            if (nuLam != 0.0F)
+            {
+                forcePair<false, true>(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
+                                       doST, nuLam, A);
+            }
+            else
+            {
+                forcePair(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam,
+                          A);
+            }
         }
     });
     return A;
@@ -1192,7 +1212,7 @@ __device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDe
 // the boundary terms, stores acc and rho, then the second half kick.
 __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, const SceneState& st, const CellBox& b,
                                             const F4& q, const F4& vi, const F4& ri, float epsH2, bool doVisc,
-                                            bool doST, uint32_t t, const PairAcc& A)
+                                            bool doST, float nuLam, uint32_t t, const PairAcc& A)
 {
     float ax = st.gEff[0];
     float ay = st.gEff[1];
@@ -1205,6 +1225,12 @@ __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, co
         ax = ax + A.avx;
         ay = ay + A.avy;
         az = az + A.avz;
+    }
+    if (nuLam != 0.0F)
+    {
+        ax = ax + A.amx;
+        ay = ay + A.amy;
+        az = az + A.amz;
     }
     if (doST)
     {
@@ -1259,10 +1285,10 @@ __device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
 // the mass and smoothing length are gathered too.
 // With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^40]) the quotient
 // d / h0 is taken through the reciprocal of h0.
-template <bool kUniform, bool kSharedH = false>
+template <bool kUniform, bool kSharedH = false, bool kLam = false>
 __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
-                                          bool doST, PairAcc& A)
+                                          bool doST, float nuLam, PairAcc& A)
 {
     const float m0 = sc.m0;
     const float h0 = sc.h0;
@@ -1298,8 +1324,8 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u)
         {
-            forcePair<kSharedH>(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x,
-                                kUniform ? wd0 : hm[u].w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A, y0,
+            forcePair<kSharedH, kLam>(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x,
+                                kUniform ? wd0 : hm[u].w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A, y0,
                                 j[u] != t);
         }
     }
@@ -1310,13 +1336,15 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
         {
             if (kUniform)
             {
-                forcePair<kSharedH>(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A,
+                forcePair<kSharedH, kLam>(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
+                                          nuLam, A,
                                     y0);
             }
             else
             {
                 const F4 h1 = c.shm[jj];
-                forcePair(c.fa[jj], c.fb[jj], h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+                forcePair<false, kLam>(c.fa[jj], c.fb[jj], h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
+                                       doST, nuLam, A);
             }
         }
     }
@@ -1344,20 +1372,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const float prri = ri.y;
         const bool doVisc = sc.alpha != 0.0F;
         const bool doST = sc.sigma != 0.0F;
+        const float nuLam = sc.nuLam;
         const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
-        PairAcc A {0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
+        PairAcc A {};
         if ((sc.flags & kSceneUniformHM) != 0U)
         {
             // (the flag is set only for h0 in [2^-40, 2^40])
-            forceList<true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            if (nuLam != 0.0F)
+            {
+                forceList<true, true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+            }
+            else
+            {
+                forceList<true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+            }
         }
         else
         {
-            forceList<false>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, A);
+            if (nuLam != 0.0F)
+            {
+                forceList<false, false, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+            }
+            else
+            {
+                forceList<false>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+            }
         }
-        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, t, A);
+        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
     }
 }
 
@@ -1376,11 +1419,12 @@ __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
         const float hi = ri.z;
         const bool doVisc = sc.alpha != 0.0F;
         const bool doST = sc.sigma != 0.0F;
+        const float nuLam = sc.nuLam;
         const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
-        const PairAcc A = forceWalk(c, sc, st.lostEnd, b, q, vi, ri.x, ri.y, nuNum, epsH2, doVisc, doST, t);
-        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, t, A);
+        const PairAcc A = forceWalk(c, sc, st.lostEnd, b, q, vi, ri.x, ri.y, nuNum, epsH2, doVisc, doST, nuLam, t);
+        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
     }
 }
 

@@ -12,7 +12,8 @@
 //   asph_run --template FILE       configuration with every default
 //
 // Extension fields (not in the reference schema, optional): in
-// simulationParameters "taitGamma", "sloshAmplitude", "sloshPeriod", "flags";
+// simulationParameters "taitGamma", "sloshAmplitude", "sloshPeriod", "flags",
+// "laminarViscosity";
 // in refinementParameters a "zone" object and criterionType "zone".
 #include <chrono>
 #include <cstdio>
@@ -84,6 +85,10 @@ Config readConfig(const asph::json::Value& root)
     if (sp.has("flags"))
     {
         p.flags = static_cast<uint32_t>(sp.at("flags").u64());
+    }
+    if (sp.has("laminarViscosity"))
+    {
+        p.laminarViscosity = sp.at("laminarViscosity").f32();
     }
     if (root.has("refinementParameters") && !root.at("refinementParameters").isNull())
     {
@@ -181,7 +186,8 @@ std::string writeConfig(const Config& c)
       << "        \"taitGamma\": " << num(p.taitGamma) << ",\n"
       << "        \"sloshAmplitude\": " << num(p.sloshAmplitude) << ",\n"
       << "        \"sloshPeriod\": " << num(p.sloshPeriod) << ",\n"
-      << "        \"flags\": " << p.flags << "\n    },\n";
+      << "        \"flags\": " << p.flags << ",\n"
+      << "        \"laminarViscosity\": " << num(p.laminarViscosity) << "\n    },\n";
     if (!c.refinement)
     {
         o << "    \"refinementParameters\": null\n}\n";

@@ -71,6 +71,12 @@ typedef struct asph_parameters {
     float slosh_amplitude;
     float slosh_period;
     uint32_t flags;             /* extension: ASPH_FLAG_* */
+    /* extension: kinematic viscosity nu (m^2/s) of a Morris laminar term
+       (Morris, Fox & Zhu 1997, eq. 6, with mu = rho nu), added after the
+       artificial viscosity: a_i += sum_j m_j nu (rho_i + rho_j) / (rho_i rho_j)
+       * (r_ij . grad_i W(r, h_j)) / (r^2 + 0.01 h_i^2) * v_ij. 0 = off (the
+       reference has no such term). */
+    float laminar_viscosity;
 } asph_parameters;
 
 /* RefinementParameters::Criterion (RefinementParameters.cuh:82-87) + extension */

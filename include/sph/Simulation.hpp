@@ -269,6 +269,7 @@ public:
         float sloshAmplitude = 0.F;
         float sloshPeriod = 0.F;
         uint32_t flags = 0;  // ASPH_FLAG_*
+        float laminarViscosity = 0.F;  // Morris laminar term's nu (0 = off)
 
         [[nodiscard]] asph_parameters toC() const
         {
@@ -290,6 +291,7 @@ public:
             p.slosh_amplitude = sloshAmplitude;
             p.slosh_period = sloshPeriod;
             p.flags = flags;
+            p.laminar_viscosity = laminarViscosity;
             return p;
         }
     };

@@ -312,6 +312,7 @@ struct Params
     T gamma;
     T sloshA, sloshT;
     uint32_t flags;
+    T nuLam;  // extension: Morris laminar viscosity (0 = off)
 };
 
 template <class T>
@@ -665,6 +666,41 @@ public:
                     av.z = av.z - s * dir.z;
                 });
                 a = {a.x + av.x, a.y + av.y, a.z + av.z, a.w};
+            }
+
+            // Extension (no reference counterpart): Morris, Fox & Zhu (1997),
+            // J. Comput. Phys. 136, 214, eq. 6, with mu = rho nu:
+            // a_i += sum_j m_j (mu_i + mu_j) / (rho_i rho_j)
+            //        * (r_ij . grad_i W_ij) / (r_ij^2 + 0.01 h_i^2) * v_ij,
+            // r_ij . grad_i W_ij = |r_ij| dW/dr (W with h_j, as the reference's
+            // other pair terms), added after the artificial viscosity.
+            if (p.nuLam != T(0))
+            {
+                V4<T> am {};
+                forEach(fluid, pos, q, T(2) * hi, [&](uint32_t j, const V4<T>& pj) {
+                    if (j == i)
+                    {
+                        return;
+                    }
+                    const V4<T> off = sub4(q, pj);
+                    const T d2 = dot4(off, off);
+                    const T hj = h[j];
+                    const T r2 = (T(2) * hj) * (T(2) * hj);
+                    if (d2 > r2)
+                    {
+                        return;
+                    }
+                    const T d = std::sqrt(d2);
+                    const T mj = m[j];
+                    const T rhoj = rho[j];
+                    const T k = ((mj * p.nuLam) * (rhoi + rhoj)) / (rhoi * rhoj);
+                    const T s = (k * (d * wendlandDerivative(d, hj))) / (d2 + ((eps * hi) * hi));
+                    const V4<T> vd = sub4(vi, vel[j]);
+                    am.x = am.x + s * vd.x;
+                    am.y = am.y + s * vd.y;
+                    am.z = am.z + s * vd.z;
+                });
+                a = {a.x + am.x, a.y + am.y, a.z + am.z, a.w};
             }
 
             if (p.sigma != T(0))
@@ -1438,6 +1474,7 @@ void loadParams(oracle::Sim<T>& s, const asph_parameters& p, const asph_refineme
     q.sloshA = p.slosh_amplitude;
     q.sloshT = p.slosh_period;
     q.flags = p.flags;
+    q.nuLam = p.laminar_viscosity;
     auto& f = s.ref;
     f.enabled = r != nullptr && r->enabled != 0U;
     if (r != nullptr)

@@ -156,3 +156,23 @@ def test_kernel_merged_h_flag_changes_only_merged_h():
     assert f0["smoothing_radii"].max() > 1.7 * hb
     assert f1["smoothing_radii"].max() < 1.3 * hb
     assert keep.size > 0
+
+
+def test_morris_laminar_viscosity_with_refinement():
+    """The Morris term in the general (non-uniform h and m) force loop:
+    refining scene, 12 steps, bit-exact with the FP32 oracle."""
+    r = refinement("velocity", 6000, velocity=(0.03, 0.2), initial_cooldown=1, cooldown=2)
+    s, sim, ref = _adaptive_pair(r, seed=33)
+    s.params.laminar_viscosity = 5e-3
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    st = random_state(s, 34, 0.1)
+    sim.set_fluid(dict(st))
+    ref.set_fluid(dict(st))
+    counts = set()
+    for k in range(12):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        assert_bitexact(sim.fluid(), ref.fluid(), f"Morris refinement step {k + 1}")
+        counts.add(sim.get_fluid_particles_count())
+    assert len(counts) > 1  # the scene refined: non-uniform h and m took the general loop

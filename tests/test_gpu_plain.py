@@ -283,3 +283,22 @@ def test_shared_reciprocal_division_equals_division():
     bad = C.c_uint64(123)
     assert lib.asph_debug_division_check(7, 1 << 30, C.byref(bad)) == 0, abi.last_error()
     assert bad.value == 0
+
+
+def test_morris_laminar_viscosity_bitexact():
+    """The Morris laminar term (extension, laminar_viscosity > 0) beside the
+    artificial viscosity: 40 steps of a dam break equal the FP32 oracle bit
+    for bit, and the term changes the trajectory."""
+    s = scene.dam_break((10, 14, 8), 0.01, (0.3, 0.3, 0.2), seed=31, steps=40)
+    s.params.laminar_viscosity = 5e-3
+    sim, ref = _pair(s)
+    for k in range(s.steps):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        if k in (0, 9, 39):
+            assert_bitexact(sim.fluid(), ref.fluid(), f"Morris step {k + 1}")
+    s.params.laminar_viscosity = 0.0
+    plain = Simulation.from_scene(s)
+    for _ in range(s.steps):
+        plain.update(s.dt)
+    assert not np.array_equal(plain.fluid()["velocities"], sim.fluid()["velocities"])
