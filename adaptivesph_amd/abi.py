@@ -281,6 +281,7 @@ def _extra_sigs():
             [vp, P(Parameters), vp, u32, vp, u32, P(RefinementParameters), u32, f32, P(u32)],
         ),
         "asph_batch_remove_scene": (C.c_int, [vp, u32]),
+        "asph_batch_update_domain_walls": (C.c_int, [vp, u32, P(Domain), f32]),
         "asph_batch_step": (C.c_int, [vp, u32]),
         "asph_batch_synchronize": (C.c_int, [vp]),
         "asph_batch_step_timed": (C.c_int, [vp, P(f32)]),

@@ -80,6 +80,11 @@ class Batch:
     def remove(self, sid: int) -> None:
         self._check(self._lib.asph_batch_remove_scene(self._h, int(sid)), "remove scene")
 
+    def update_domain_walls(self, sid: int, domain: abi.Domain, spacing: float) -> None:
+        """Moves scene `sid`'s box; its walls are rebuilt on the GPU."""
+        self._check(self._lib.asph_batch_update_domain_walls(self._h, int(sid), C.byref(domain), float(spacing)),
+                    "update domain walls")
+
     def step(self, steps: int = 1) -> None:
         self._check(self._lib.asph_batch_step(self._h, int(steps)), "step")
 

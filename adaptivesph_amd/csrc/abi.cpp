@@ -422,6 +422,15 @@ int asph_batch_remove_scene(asph_batch* batch, uint32_t scene_id)
     return guard([&] { engineOf(batch).removeScene(scene_id); });
 }
 
+int asph_batch_update_domain_walls(asph_batch* batch, uint32_t scene_id, const asph_domain* domain,
+                                   float particle_spacing)
+{
+    return guard([&] {
+        need(domain, "domain");
+        engineOf(batch).updateDomainWalls(scene_id, *domain, particle_spacing);
+    });
+}
+
 int asph_batch_step(asph_batch* batch, uint32_t steps)
 {
     return guard([&] { engineOf(batch).step(steps); });

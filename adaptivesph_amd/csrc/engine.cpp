@@ -939,6 +939,10 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
 
 void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacing)
 {
+    if (id >= scenes_.size() || !scenes_[id].active)
+    {
+        throw Error(ASPH_ERR_INVALID, "no such scene");
+    }
     if (!(spacing > 0.0F))
     {
         throw Error(ASPH_ERR_INVALID, "particle spacing must be positive");

@@ -42,6 +42,13 @@ int asph_batch_add_scene(asph_batch* batch, const asph_parameters* params, const
                          uint32_t* scene_id);
 int asph_batch_remove_scene(asph_batch* batch, uint32_t scene_id);
 
+/* asph_update_domain_walls for one scene of the batch (between steps): the
+   scene's box moves or resizes and its walls are rebuilt on the GPU; the
+   other scenes are untouched (Simulation::updateDomain, SphSimulation.cu:
+   122-134, per scene). */
+int asph_batch_update_domain_walls(asph_batch* batch, uint32_t scene_id, const asph_domain* domain,
+                                   float particle_spacing);
+
 /* Enqueue `steps` steps of every scene; returns without waiting. */
 int asph_batch_step(asph_batch* batch, uint32_t steps);
 int asph_batch_synchronize(asph_batch* batch);
