@@ -207,8 +207,9 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
-        // 16, compaction copies 64, merge list capture 4, sorted slot 4
-        perParticle += 24ULL + 16ULL + 64ULL + 4ULL + 4ULL;
+        // 16, compaction copies 64, merge list capture 4, sorted slot 4, query
+        // order 8
+        perParticle += 24ULL + 16ULL + 64ULL + 4ULL + 4ULL + 8ULL;
     }
     // boundary slot: position + psi 16, viscosity 4, chunk box 32; cell: count,
     // offset and boundary offset 12, boundary box 32, wall distance bound 4.
@@ -300,6 +301,8 @@ Engine::Engine(const EngineConfig& cfg)
             c.cvalsB = alloc<uint32_t>(P);
             c.cand = alloc<uint32_t>(P);
             c.slotOf = alloc<uint32_t>(P);
+            c.qperm = alloc<uint32_t>(P);
+            c.qpos = alloc<uint32_t>(P);
             c.removal = alloc<uint32_t>(P);
             c.keep = alloc<uint32_t>(P);
             c.rprefix = alloc<uint32_t>(P);
@@ -1164,6 +1167,10 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
                  s);
     mark();
     launchGather(c, lc, s);
+    if (c.qperm != nullptr)
+    {
+        launchQueryOrder(c, s);
+    }
     mark();
     launchNeighbors(c, lc, s);
     mark();

@@ -112,6 +112,14 @@ struct Ctx
     uint32_t* csortedVals;
     uint32_t* cand;
     uint32_t* slotOf;  // [pCap] sorted slot of each canonical particle (written by kGather; adaptive engines)
+    // Query order of the list builder and the force kernel (adaptive engines;
+    // null otherwise, meaning the sorted order): query position p handles
+    // sorted slot qperm[p], and slot t's neighbour list sits at query
+    // position qpos[t]. Within each cell the slots are grouped by sub-cell, so
+    // the 64 queries of a wave are near each other even where refinement
+    // packs a cell with daughters in index order.
+    uint32_t* qperm;
+    uint32_t* qpos;
     uint32_t* removal;
     uint32_t* keep;
     uint32_t* rprefix;
@@ -136,6 +144,7 @@ struct LaunchCfg
 void launchPrep(const Ctx& c, hipStream_t s);
 void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
+void launchQueryOrder(const Ctx& c, hipStream_t s);
 void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s);
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s);

@@ -333,6 +333,81 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
     }
 }
 
+// Query order (adaptive engines): within each cell, the slots grouped by
+// their sub-cell (2 bits per axis, Morton order), one wave per cell. Any
+// grouping gives the same lists and sums (each lane's list depends on its
+// own query only); grouping keeps a wave's queries close together, so its
+// candidate filter keeps few survivors even in refined cells, whose
+// particles are in index order (daughters appended per split).
+__global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
+{
+    __shared__ uint32_t bins[4][64];
+    const uint32_t lane = threadIdx.x & 63U;
+    const uint32_t w = threadIdx.x >> 6U;
+    const uint32_t waves = gridDim.x * 4U;
+    // cellOff holds entries 0 .. cExtent - 1 (the last one an end entry):
+    // the ranges of cells 0 .. cExtent - 2 cover every sorted slot once
+    for (uint32_t cell = blockIdx.x * 4U + w; cell + 1U < c.cExtent; cell += waves)
+    {
+        const uint32_t lo = c.cellOff[cell];
+        const uint32_t hi = c.cellOff[cell + 1U];
+        if (hi - lo <= 64U)
+        {
+            // one wave's worth: the sorted order
+            if (lane < hi - lo)
+            {
+                c.qperm[lo + lane] = lo + lane;
+                c.qpos[lo + lane] = lo + lane;
+            }
+            continue;
+        }
+        uint32_t* bin = bins[w];
+        bin[lane] = 0U;
+        const SceneDev& sc = c.scenes[c.sscene[lo]];
+        auto subOf = [&](uint32_t t) {
+            const F4 q = c.s0[t];
+            const float u[3] = {(q.x - sc.dmin[0]) / sc.cs[0], (q.y - sc.dmin[1]) / sc.cs[1],
+                                (q.z - sc.dmin[2]) / sc.cs[2]};
+            uint32_t m = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+            {
+                const float f = u[a] - floorf(u[a]);
+                const uint32_t k = min(static_cast<uint32_t>(fmaxf(f, 0.0F) * 4.0F), 3U);
+                m |= ((k & 1U) << a) | ((k >> 1U) << (a + 3));
+            }
+            return m;
+        };
+        for (uint32_t t0 = lo; t0 < hi; t0 += 64U)
+        {
+            const uint32_t t = t0 + lane;
+            if (t < hi)
+            {
+                atomicAdd(&bin[subOf(t)], 1U);
+            }
+        }
+        // exclusive scan of the 64 bins across the wave
+        const uint32_t count = bin[lane];
+        uint32_t incl = count;
+        for (uint32_t off = 1; off < 64U; off <<= 1U)
+        {
+            const uint32_t y = __shfl_up(incl, off, 64);
+            incl += lane >= off ? y : 0U;
+        }
+        bin[lane] = lo + incl - count;
+        for (uint32_t t0 = lo; t0 < hi; t0 += 64U)
+        {
+            const uint32_t t = t0 + lane;
+            if (t < hi)
+            {
+                const uint32_t p = atomicAdd(&bin[subOf(t)], 1U);
+                c.qperm[p] = t;
+                c.qpos[t] = p;
+            }
+        }
+    }
+}
+
 // True unless every member of the group is provably farther from q than its
 // own support 2h (conservative by a relative margin far above FP32 rounding).
 __device__ __forceinline__ bool groupMayHit(const F4& b0, const F4& b1, const F4& q)
@@ -777,7 +852,7 @@ __device__ __forceinline__ float quotientH0(float d, float h0, float y0)
 // visiting order. UB and GB are the entries whose data are loaded together in
 // the uniform-h/m and general loops.
 template <uint32_t UB, uint32_t GB>
-__device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q)
+__device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, uint32_t lp, uint32_t cnt, const F4& q)
 {
     float rho = 0.0F;
     if ((sc.flags & kSceneUniformHM) != 0U)
@@ -797,7 +872,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < UB; u += 4U)
             {
-                const uint4 v = loadChunk(listAt(c, t, e + u));
+                const uint4 v = loadChunk(listAt(c, lp, e + u));
                 j[u] = v.x;
                 j[u + 1U] = v.y;
                 j[u + 2U] = v.z;
@@ -820,7 +895,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
         }
         for (; e < cnt; ++e)
         {
-            const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, t, e));
+            const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, lp, e));
             const float dx = o.x - q.x;
             const float dy = o.y - q.y;
             const float dz = o.z - q.z;
@@ -838,7 +913,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < GB; u += 4U)
             {
-                const uint4 v = loadChunk(listAt(c, t, e + u));
+                const uint4 v = loadChunk(listAt(c, lp, e + u));
                 j[u] = v.x;
                 j[u + 1U] = v.y;
                 j[u + 2U] = v.z;
@@ -862,7 +937,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
         }
         for (; e < cnt; ++e)
         {
-            const uint32_t jj = *listAt(c, t, e);
+            const uint32_t jj = *listAt(c, lp, e);
             const F4 o = c.s0[jj];
             const F4 hm = c.shm[jj];
             const float dx = o.x - q.x;
@@ -902,7 +977,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
     if (tw < nPad)
     {
         const bool valid = tw < n;
-        const uint32_t t = valid ? tw : n - 1U;
+        // query position pq handles sorted slot t; the list sits at pq
+        const uint32_t pq = valid ? tw : n - 1U;
+        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
         const uint32_t s = c.sscene[t];
         const uint32_t sw = __builtin_amdgcn_readfirstlane(s);
         const bool oneScene = __ballot((valid && s != sw) ? 1 : 0) == 0ULL;
@@ -918,7 +995,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         uint32_t done = 0;
         // the thread's next chunk (listChunk): chunk rows of a wave's block
         // are 64 chunks apart
-        uint4* cptr = reinterpret_cast<uint4*>(c.list) + listChunk(c, t, 0U);
+        uint4* cptr = reinterpret_cast<uint4*>(c.list) + listChunk(c, pq, 0U);
         constexpr uint32_t cstep = 64U;
         auto push = [&](bool hit, uint32_t j) {
             stage[(cnt & (kRing - 1U)) * BS + tid] = j;
@@ -967,7 +1044,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
                 // the density over the list just written (its entries and the
                 // neighbours' records are still in cache), then the boundary
                 // term, the pressure and the force records
-                const float rho = listDensity<kListDensityUniformBatch, kListDensityBatch>(c, sc, t, cnt, q);
+                const float rho = listDensity<kListDensityUniformBatch, kListDensityBatch>(c, sc, pq, cnt, q);
                 finishDensity(c, sc, b, q, hi, t, rho);
             }
         }
@@ -1286,7 +1363,8 @@ __device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
 // With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^40]) the quotient
 // d / h0 is taken through the reciprocal of h0.
 template <bool kUniform, bool kSharedH = false, bool kLam = false>
-__device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t cnt, const F4& q,
+__device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t lp, uint32_t cnt,
+                                          const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
                                           bool doST, float nuLam, PairAcc& A)
 {
@@ -1305,7 +1383,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; u += 4U)
         {
-            const uint4 v = loadChunk(listAt(c, t, e + u));
+            const uint4 v = loadChunk(listAt(c, lp, e + u));
             j[u] = v.x;
             j[u + 1U] = v.y;
             j[u + 2U] = v.z;
@@ -1331,7 +1409,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     }
     for (; e < cnt; ++e)
     {
-        const uint32_t jj = *listAt(c, t, e);
+        const uint32_t jj = *listAt(c, lp, e);
         if (jj != t)
         {
             if (kUniform)
@@ -1354,8 +1432,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 {
     const uint32_t n = *c.nTotal;
     const uint32_t K = c.listCap;
-    for (uint32_t t = xcdBlock(n) * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    // query positions pq, in the list builder's order (slot t's list at pq)
+    for (uint32_t pq = xcdBlock(n) * blockDim.x + threadIdx.x; pq < n; pq += gridDim.x * blockDim.x)
     {
+        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
         const uint32_t cnt = c.ncount[t];
         if (cnt > K)
         {
@@ -1382,22 +1462,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             // (the flag is set only for h0 in [2^-40, 2^40])
             if (nuLam != 0.0F)
             {
-                forceList<true, true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+                forceList<true, true, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
             else
             {
-                forceList<true, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+                forceList<true, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
         }
         else
         {
             if (nuLam != 0.0F)
             {
-                forceList<false, false, true>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+                forceList<false, false, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
             else
             {
-                forceList<false>(c, sc, t, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+                forceList<false>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
         }
         finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
@@ -1798,6 +1878,7 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
         const uint32_t cnt = c.ncount[ti];
         if (cnt <= c.listCap)
         {
+            const uint32_t lpi = c.qperm != nullptr ? c.qpos[ti] : ti;  // the list's query position
             // Every hit of the walk over the cells within h_i is in the
             // particle's neighbour list of this step (d2 < h_j^2 < (2 h_j)^2,
             // cells within h_i inside those within 2 h_i), and the list holds
@@ -1806,7 +1887,7 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
             const CellBox b = cellBox(sc, q.x, q.y, q.z, hh);
             for (uint32_t e = 0; e < cnt; ++e)
             {
-                const uint32_t slot = *listAt(c, ti, e);
+                const uint32_t slot = *listAt(c, lpi, e);
                 const float d2 = visit(slot);
                 if (d2 >= 0.0F)
                 {
@@ -2278,6 +2359,13 @@ void launchKickDriftKey(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     hipLaunchKernelGGL(kKickDriftKey, dim3(gridFor(c.nMax, lc.blockStream)), dim3(lc.blockStream), 0, s, c);
 }
 
+
+void launchQueryOrder(const Ctx& c, hipStream_t s)
+{
+    const uint32_t blocks = (c.cExtent + 3U) / 4U;
+    hipLaunchKernelGGL(kQueryOrder, dim3(blocks < kGridCap ? (blocks == 0U ? 1U : blocks) : kGridCap), dim3(256), 0, s,
+                       c);
+}
 
 void launchGather(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
