@@ -185,6 +185,28 @@ __device__ __forceinline__ float divBy(float a, float b, float y)
     return __builtin_fmaf(r1, y, q1);
 }
 
+// Correctly rounded FP32 square root as hipcc emits it for gfx950: inputs
+// below 2^-96 scaled by 2^32, s = v_sqrt(x), then s - 1 ulp or s + 1 ulp
+// chosen by the signs of fma(-(s -+ ulp), s, x), the result unscaled by
+// 2^-16, and zeros and infinities passed through. For x = +0 or x in
+// [2^-96, 2^127) the scalings and the pass-through are identities (+0 runs
+// through the corrections unchanged), so the steps below give sqrtf's bits.
+constexpr float kSqrtLo = 0x1p-96F;
+constexpr float kSqrtHi = 0x1p127F;
+
+__device__ __forceinline__ float sqrtRanged(float x)
+{
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __uint_as_float(__float_as_uint(s) - 1U);
+    const float sp = __uint_as_float(__float_as_uint(s) + 1U);
+    float t = (__builtin_fmaf(-sm, s, x) <= 0.0F) ? sm : s;
+    t = (__builtin_fmaf(-sp, s, x) > 0.0F) ? sp : t;
+    return t;
+}
+
+// x in the domain of sqrtRanged
+__device__ __forceinline__ bool sqrtInRange(float x) { return x == 0.0F || (x >= kSqrtLo && x < kSqrtHi); }
+
 // wendlandKernel / wendlandDerivativeKernel (WendlandKernel.cuh:39-52, 69-84)
 // given q = d / h and the per-particle factor; wendlandC and wendlandGradC
 // below are these with q = d / h

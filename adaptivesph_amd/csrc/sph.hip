@@ -832,14 +832,16 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
     c.fb[t] = F4 {v.x, v.y, v.z, rho};
 }
 
-// d / h0 for a distance d >= +0 (a square root) through y0 = divRecip(h0),
-// h0 in [2^-40, 2^40]; the division where d is outside +0 or [2^-40, 2^40]
-__device__ __forceinline__ float quotientH0(float d, float h0, float y0)
+// sqrtf(d2) / h0 through sqrtRanged and y0 = divRecip(h0), h0 in
+// [2^-40, 2^40]: for d2 = +0 or d2 in [2^-80, 2^80] the root is +0 or lies in
+// [2^-40, 2^40], inside both functions' domains; otherwise (rare) sqrtf and
+// the division
+__device__ __forceinline__ float quotientH0(float d2, float h0, float y0)
 {
-    float q = divBy(d, h0, y0);
-    if (!(d <= kDivHi && (d >= kDivLo || d == 0.0F)))
+    float q = divBy(sqrtRanged(d2), h0, y0);
+    if (!(d2 == 0.0F || (d2 >= 0x1p-80F && d2 <= 0x1p80F)))
     {
-        q = d / h0;
+        q = sqrtf(d2) / h0;
     }
     return q;
 }
@@ -889,8 +891,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
                 const float dx = o[u].x - q.x;
                 const float dy = o[u].y - q.y;
                 const float dz = o[u].z - q.z;
-                const float d = sqrtf(dot3w(dx, dy, dz, dx, dy, dz));
-                rho = rho + m0 * wendlandQ(quotientH0(d, h0, y0), c3);
+                rho = rho + m0 * wendlandQ(quotientH0(dot3w(dx, dy, dz, dx, dy, dz), h0, y0), c3);
             }
         }
         for (; e < cnt; ++e)
@@ -899,7 +900,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
             const float dx = o.x - q.x;
             const float dy = o.y - q.y;
             const float dz = o.z - q.z;
-            rho = rho + m0 * wendlandQ(quotientH0(sqrtf(dot3w(dx, dy, dz, dx, dy, dz)), h0, y0), c3);
+            rho = rho + m0 * wendlandQ(quotientH0(dot3w(dx, dy, dz, dx, dy, dz), h0, y0), c3);
         }
     }
     else
@@ -1098,17 +1099,19 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
     const float oy = q.y - o.y;
     const float oz = q.z - o.z;
     const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
-    const float d = sqrtf(d2);
+    float d;
     float dx = 0.0F;
     float dy = 1.0F;
     float dz = 0.0F;
     float qh;
-    // every component in [2^-40, 2^39] puts d in [2^-40, 2^40] too: then the
-    // four quotients (three over d, one over hj) go through reciprocals
+    // every component in [2^-40, 2^39] puts d2 in [2^-80, 2^80] and d in
+    // [2^-40, 2^40]: then the root goes through sqrtRanged and the four
+    // quotients (three over d, one over hj) through reciprocals
     const float lo = fminf(fminf(fabsf(ox), fabsf(oy)), fabsf(oz));
     const float hi = fmaxf(fmaxf(fabsf(ox), fabsf(oy)), fabsf(oz));
     if ((lo >= kDivLo && hi <= 0x1p39F) || !use)
     {
+        d = sqrtRanged(d2);
         const float yd = divRecip(d);
         dx = divBy(ox, d, yd);
         dy = divBy(oy, d, yd);
@@ -1117,6 +1120,7 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
     }
     else
     {
+        d = sqrtf(d2);
         if (d > 0.0F)
         {
             dx = ox / d;
@@ -2496,7 +2500,9 @@ void launchDensityInfo(const Ctx& c, uint32_t scene, uint32_t count, float thres
 // Compares divBy(a, b, divRecip(b)) with the division a / b on `count`
 // pseudo-random pairs: a = +0 or +-m * 2^ea, b = m' * 2^eb with exponents
 // uniform in [-40, 39] and random 23-bit mantissas, plus, every fourth pair,
-// a in [0, 2.2 b] (the kernels' d / h range). Counts the pairs whose bits differ.
+// a in [0, 2.2 b] (the kernels' d / h range); and sqrtRanged(x) with sqrtf(x)
+// for x = +0 or m * 2^e, e uniform in [-96, 126]. Counts the results whose
+// bits differ.
 // SplitMix64 finaliser
 __device__ __forceinline__ uint64_t checkMix(uint64_t x)
 {
@@ -2541,6 +2547,11 @@ __global__ __launch_bounds__(256) void kDivisionCheck(uint64_t seed, uint64_t co
         const float fast = divBy(a, b, divRecip(b));
         const float ieee = a / b;
         local += (__float_as_uint(fast) != __float_as_uint(ieee)) ? 1U : 0U;
+        // sqrtRanged against sqrtf: x = +0 or exponents uniform in [-96, 126]
+        const uint64_t r2 = checkMix(seed ^ (k * 2ULL + 0x5A5A5A5AULL));
+        const uint32_t ex = static_cast<uint32_t>((r2 >> 23) % 223U) + 31U;
+        const float x = (k & 255ULL) == 253ULL ? 0.0F : __uint_as_float((ex << 23) | static_cast<uint32_t>(r2 & 0x7FFFFFU));
+        local += (__float_as_uint(sqrtRanged(x)) != __float_as_uint(sqrtf(x))) ? 1U : 0U;
     }
     if (local != 0U)
     {

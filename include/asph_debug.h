@@ -63,8 +63,10 @@ int asph_debug_refinement(asph_simulation* sim, asph_refinement_trace* trace,
 
 /* Checks, on `count` pseudo-random operand pairs (seeded) with magnitudes in
    [2^-40, 2^40], that the kernels' division through a shared reciprocal gives
-   the same bits as the FP32 division; *mismatches receives the number of
-   pairs that differ (0 expected). Uses the current device; synchronous. */
+   the same bits as the FP32 division, and on `count` pseudo-random values
+   (+0 or [2^-96, 2^127)) that their range-restricted square root gives the
+   bits of sqrtf; *mismatches receives the number of results that differ
+   (0 expected). Uses the current device; synchronous. */
 int asph_debug_division_check(uint64_t seed, uint64_t count, uint64_t* mismatches);
 
 #ifdef __cplusplus

@@ -273,10 +273,12 @@ def test_box_filled_with_active_walls():
 
 
 def test_shared_reciprocal_division_equals_division():
-    """The density and force loops divide by a shared reciprocal (the steps
-    of the correctly rounded division without its range scaling); on 2^30
-    pseudo-random operand pairs with magnitudes in [2^-40, 2^40], plus +0 and
-    the d / h range, every quotient has the bits of a / b."""
+    """The density and force loops divide by a shared reciprocal and take
+    square roots without range scaling (the steps of the correctly rounded
+    operations without their scalings); on 2^30 pseudo-random operand pairs
+    with magnitudes in [2^-40, 2^40], plus +0 and the d / h range, every
+    quotient has the bits of a / b, and on 2^30 values in [2^-96, 2^127) and
+    +0 every root has the bits of sqrtf."""
     import ctypes as C
 
     lib = abi.load_library()
