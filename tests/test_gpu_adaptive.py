@@ -176,3 +176,27 @@ def test_morris_laminar_viscosity_with_refinement():
         assert_bitexact(sim.fluid(), ref.fluid(), f"Morris refinement step {k + 1}")
         counts.add(sim.get_fluid_particles_count())
     assert len(counts) > 1  # the scene refined: non-uniform h and m took the general loop
+
+
+def test_cfg3_sloshing_tank_at_full_size():
+    """The 500,000-particle adaptive sloshing tank (zone refinement every
+    step, kernel-normalised merged h) for 4 steps: refinement decisions and
+    the whole state equal the FP32 oracle's bits at full size, where refined
+    cells hold hundreds of particles (query order, list capacity 640, merge
+    search over the lists, criterion sorts over 500k keys)."""
+    s = scene.cfg3()
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    merged = split = 0
+    for step in range(1, 5):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        ta, tb = sim.refinement_trace(), ref.refinement_trace()
+        _compare_traces(ta, tb, f"cfg3 step {step}")
+        merged += ta["merge_pairs"]
+        split += ta["split_performed"]
+        assert sim.get_fluid_particles_count() == ref.count()
+    assert_bitexact(sim.fluid(), ref.fluid(), "cfg3 step 4")
+    assert merged > 1000 and split > 100, (merged, split)
+    sim.close()
+    ref.close()
