@@ -126,3 +126,18 @@ def test_list_overflow_paths_match_alone(list_capacity):
     for s, sid in zip(scenes, ids):
         assert_bitexact(b.fluid(sid), _alone(s, steps), f"{s.name} list capacity {list_capacity}")
     b.close()
+
+
+def test_benchmark_scene_mix_equals_alone():
+    """Twelve scenes of the batched benchmark's generator (dam breaks and
+    sloshing tanks, half refining, kernel-normalised merged h; sizes capped
+    at 30k particles) stepped 15 times in one graph-replayed batch: every
+    scene equals itself run alone."""
+    scenes = [scene.random_scene(1000 + i, n_max=30_000, steps=15, merged_h="kernel") for i in range(12)]
+    assert {s.meta["kind"] for s in scenes} == {"dam", "slosh"} and len({s.adaptive for s in scenes}) == 2
+    b = Batch.for_scenes(scenes, graph=True)
+    ids = [b.add(s) for s in scenes]
+    b.step(15)
+    for s, sid in zip(scenes, ids):
+        assert_bitexact(b.fluid(sid), _alone(s, 15), f"{s.name}")
+    b.close()
