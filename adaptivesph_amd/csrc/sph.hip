@@ -2412,8 +2412,12 @@ void launchDensity(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 
 void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
 {
+    // one thread per slot, uncapped: particles differ widely in cost (a
+    // refined region's hundreds of neighbours), and many small workgroups let
+    // the dispatcher balance them where a capped grid would give each wave a
+    // fixed share
     const uint32_t b = lc.blockNeighbor;
-    hipLaunchKernelGGL(kForce, dim3(gridFor(c.nMax, b)), dim3(b), 0, s, c);
+    hipLaunchKernelGGL(kForce, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
     hipLaunchKernelGGL(kForceOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);
 }
 
