@@ -370,7 +370,7 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--latency-steps", type=int, default=50)
-    ap.add_argument("--rate", type=float, default=2.4, help="serve: requests per second")
+    ap.add_argument("--rate", type=float, default=2.8, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: device memory budget")
     ap.add_argument("--list-capacity", type=int, default=0,
