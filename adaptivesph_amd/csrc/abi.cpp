@@ -3,6 +3,7 @@
 // records the message for asph_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -193,6 +194,20 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
         cfg.cellCapacity = g.cells() + 1U;
         cfg.sceneSlots = 1;
         cfg.listCap = asph::defaultListCap(adaptive);
+        if (adaptive)
+        {
+            // the list of a very large capacity may not take more than a
+            // quarter of the free device memory (never below the plain
+            // default; particles past the capacity take the walking path)
+            size_t freeBytes = 0;
+            size_t totalBytes = 0;
+            if (hipMemGetInfo(&freeBytes, &totalBytes) == hipSuccess && cap > 0U)
+            {
+                const uint64_t fit = static_cast<uint64_t>(freeBytes) / 4U / (4ULL * cap);
+                const uint64_t k = std::max<uint64_t>(asph::kDefaultListCapPlain, fit & ~3ULL);
+                cfg.listCap = static_cast<uint32_t>(std::min<uint64_t>(cfg.listCap, k));
+            }
+        }
         cfg.adaptive = adaptive;
         cfg.useGraph = false;
         auto sim = std::make_unique<asph_simulation>();
