@@ -301,6 +301,7 @@ Engine::Engine(const EngineConfig& cfg)
             c.cvalsB = alloc<uint32_t>(P);
             c.cand = alloc<uint32_t>(P);
             c.slotOf = alloc<uint32_t>(P);
+            c.critCount = alloc<uint32_t>(1);
             c.qperm = alloc<uint32_t>(P);
             c.qpos = alloc<uint32_t>(P);
             c.removal = alloc<uint32_t>(P);
@@ -1187,7 +1188,7 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
     {
         launchMergeCriteria(c, lc, s);
         mark();
-        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
+        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.critCount, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
                      s);
         mark();
         launchMergeSelect(c, s);
@@ -1205,7 +1206,7 @@ void Engine::enqueueStep(bool refine, std::vector<hipEvent_t>* marks)
         launchCopyBack(c, lc, s);
         mark();
         launchSplitCriteria(c, lc, s);
-        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.nTotal, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
+        radixSortU64(c.ckeysA, c.cvalsA, c.ckeysB, c.cvalsB, c.critCount, c.nMax, 32 + sceneBits_, hist_, digitTotals_,
                      s);
         launchSplitSelect(c, s);
         launchSplitApply(c, lc, s);

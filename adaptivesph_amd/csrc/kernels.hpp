@@ -112,6 +112,10 @@ struct Ctx
     uint32_t* csortedVals;
     uint32_t* cand;
     uint32_t* slotOf;  // [pCap] sorted slot of each canonical particle (written by kGather; adaptive engines)
+    // device: criterion keys of the current refinement phase (merge: set by
+    // kPrep; split: by kAfterMerge); the participating scenes' keys are laid
+    // out back to back from each scene's segStart
+    uint32_t* critCount;
     // Query order of the list builder and the force kernel (adaptive engines;
     // null otherwise, meaning the sorted order): query position p handles
     // sorted slot qperm[p], and slot t's neighbour list sits at query

@@ -69,6 +69,11 @@ __device__ uint32_t blockScan256(uint32_t v, uint32_t* lds, uint32_t* total)
     return base + x - v;
 }
 
+__device__ __forceinline__ bool mergeParticipant(const SceneDev& sc, const SceneState& st)
+{
+    return (sc.flags & kSceneActive) != 0U && st.refineNow != 0U && st.countBefore > 1U;
+}
+
 // ------------------------------------------------------------------ prep --
 // Per step: dense offsets of the scenes, simulated time, sloshing gravity,
 // refinement-zone centre and the refinement cadence
@@ -153,10 +158,34 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
         }
         carry += tot;
     }
+    // merge layout: the criterion keys of the merging scenes back to back, in
+    // scene order, from each scene's segStart (findTopParticlesToMerge sorts
+    // only these)
+    uint32_t mcarry = 0;
+    for (uint32_t s0 = 0; s0 < c.sceneSlots; s0 += 256)
+    {
+        const uint32_t s = s0 + threadIdx.x;
+        uint32_t mc = 0;
+        if (s < c.sceneSlots && mergeParticipant(c.scenes[s], c.states[s]))
+        {
+            mc = c.states[s].countBefore;
+        }
+        uint32_t tot = 0;
+        const uint32_t ex = blockScan256(mc, lds, &tot);
+        if (mc > 0U)
+        {
+            c.states[s].segStart = mcarry + ex;
+        }
+        mcarry += tot;
+    }
     if (threadIdx.x == 0)
     {
         *c.nTotal = carry;
         *c.overflowCount = 0U;
+        if (c.critCount != nullptr)
+        {
+            *c.critCount = mcarry;
+        }
     }
 }
 
@@ -1762,10 +1791,6 @@ __device__ __forceinline__ void countAdd(uint32_t* ctr, uint32_t s, bool pred)
     }
 }
 
-__device__ __forceinline__ bool mergeParticipant(const SceneDev& sc, const SceneState& st)
-{
-    return (sc.flags & kSceneActive) != 0U && st.refineNow != 0U && st.countBefore > 1U;
-}
 
 // getCriterionValues (AdaptiveAlgorithm.cuh:40-54) for the merge criteria;
 // keys (scene, value ascending) for the stable sort of findTopParticlesToMerge.
@@ -1779,8 +1804,6 @@ __global__ __launch_bounds__(256) void kMergeCriteria(Ctx c)
         SceneState& st = c.states[s];
         if (!mergeParticipant(sc, st))
         {
-            c.ckeysA[d] = ~0ULL;
-            c.cvalsA[d] = 0U;
             continue;
         }
         const uint32_t l = d - st.denseOff;
@@ -1788,8 +1811,9 @@ __global__ __launch_bounds__(256) void kMergeCriteria(Ctx c)
         c.removal[d] = 0U;
         c.keep[d] = 0U;
         const float v = mergeCriterion(c, sc, st, sc.pOff + l);
-        c.ckeysA[d] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(orderedBits(v));
-        c.cvalsA[d] = l;
+        const uint32_t k = st.segStart + l;  // the merge layout of kPrep
+        c.ckeysA[k] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(orderedBits(v));
+        c.cvalsA[k] = l;
         countAdd(&st.positives, s, v > 0.0F);
     }
 }
@@ -1831,7 +1855,7 @@ __global__ void kCopyMergeList(Ctx c)
 // the first strictly closer hit in walk order wins.
 __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
 {
-    const uint32_t n = *c.nTotal;
+    const uint32_t n = *c.critCount;  // the sorted merge keys
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
         const uint64_t key = c.csortedKeys[t];
@@ -2042,6 +2066,7 @@ __global__ void kAfterMerge(Ctx c)
     {
         return;
     }
+    uint32_t run = 0;
     for (uint32_t s = 0; s < c.sceneSlots; ++s)
     {
         const SceneDev& sc = c.scenes[s];
@@ -2060,7 +2085,14 @@ __global__ void kAfterMerge(Ctx c)
         st.countAfterMerge = st.countBefore - removed;
         st.count = st.countAfterMerge;
         st.positives = 0;
+        if (removed > 0U)
+        {
+            // split layout: the splitting scenes' keys back to back
+            st.segStart = run;
+            run += st.countAfterMerge;
+        }
     }
+    *c.critCount = run;
 }
 
 __global__ __launch_bounds__(256) void kCopyBack(Ctx c)
@@ -2103,13 +2135,12 @@ __global__ __launch_bounds__(256) void kSplitCriteria(Ctx c)
         const uint32_t l = d - st.denseOff;
         if ((sc.flags & kSceneActive) == 0U || st.refineNow == 0U || st.removed == 0U || l >= st.countAfterMerge)
         {
-            c.ckeysA[d] = ~0ULL;
-            c.cvalsA[d] = 0U;
             continue;
         }
         const float v = splitCriterion(c, sc, st, sc.pOff + l);
-        c.ckeysA[d] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(~orderedBits(v));
-        c.cvalsA[d] = l;
+        const uint32_t k = st.segStart + l;  // the split layout of kAfterMerge
+        c.ckeysA[k] = (static_cast<uint64_t>(s) << 32U) | static_cast<uint64_t>(~orderedBits(v));
+        c.cvalsA[k] = l;
         countAdd(&st.positives, s, v > 0.0F);
     }
 }
@@ -2171,7 +2202,7 @@ __global__ __launch_bounds__(256) void kSplitApply(Ctx c)
     const float verts[12][3] = {{0.F, pin, inv},  {0.F, pin, -inv},  {0.F, -pin, -inv}, {0.F, -pin, inv},
                                 {inv, 0.F, pin},  {-inv, 0.F, pin},  {inv, 0.F, -pin},  {-inv, 0.F, -pin},
                                 {pin, inv, 0.F},  {pin, -inv, 0.F},  {-pin, inv, 0.F},  {-pin, -inv, 0.F}};
-    const uint32_t n = *c.nTotal;
+    const uint32_t n = *c.critCount;  // the sorted split keys
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     {
         const uint64_t key = c.csortedKeys[t];
