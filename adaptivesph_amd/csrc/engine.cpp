@@ -337,11 +337,8 @@ Engine::Engine(const EngineConfig& cfg)
             hipCheck(hipMemset(c.cand, 0, sizeof(uint32_t) * P), "hipMemset");
         }
         hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
-        // criterion keys: scene << 32 | 32 ordered value bits, sorted on
-        // 32 + sceneBits_ bits. Particles of scenes that do not refine carry
-        // ~0, whose sorted bits equal the top scene's largest value; only a
-        // NaN criterion reaches that value, and the selections take positive
-        // values, which sort before it.
+        // criterion keys: scene << 32 | 32 ordered value bits of the
+        // participating scenes only, sorted on 32 + sceneBits_ bits
         sceneBits_ = S <= 1U ? 0 : bitsFor(static_cast<uint64_t>(S) - 1U);
         scenes_.resize(S);
         c.nMax = 1;
