@@ -183,6 +183,10 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
         {
             throw asph::Error(ASPH_ERR_CAPACITY, "particle count exceeds the fluid capacity");
         }
+        if (cap > asph::kMaxParticleCapacity)
+        {
+            throw asph::Error(ASPH_ERR_CAPACITY, "particle capacity above 2^28 - 64");
+        }
         if (boundary_count > max_boundary_capacity)
         {
             throw asph::Error(ASPH_ERR_CAPACITY, "boundary count exceeds the boundary capacity");

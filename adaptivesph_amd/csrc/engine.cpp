@@ -236,6 +236,11 @@ Engine::Engine(const EngineConfig& cfg)
     {
         throw Error(ASPH_ERR_INVALID, "sceneSlots must be positive");
     }
+    if (cfg_.particleCapacity > kMaxParticleCapacity)
+    {
+        // the neighbour loops address 16-byte particle records by 32-bit byte offsets
+        throw Error(ASPH_ERR_CAPACITY, "particle capacity above 2^28 - 64");
+    }
     hipCheck(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     try
     {

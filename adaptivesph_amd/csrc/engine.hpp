@@ -63,6 +63,8 @@ void validateParams(const asph_parameters& p);
 // neighbours (h = 1.3 dx); refinement packs up to 13 daughters per parent
 // volume, and refined regions hold several hundred neighbours per particle.
 // Particles beyond the capacity take the slower walking overflow kernels.
+// Largest particle pool: record byte offsets (j * 16) must fit in 32 bits.
+constexpr uint32_t kMaxParticleCapacity = (1U << 28) - 64U;
 constexpr uint32_t kDefaultListCapPlain = 128;
 constexpr uint32_t kDefaultListCapAdaptive = 640;
 

@@ -654,6 +654,23 @@ __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint
     return c.list + listChunk(c, t, k) * kChunk + (k % kChunk);
 }
 
+// Chunk k / 4 of a list whose first chunk is `row0` (listChunk(c, t, 0)):
+// chunk rows lie 64 chunks apart, so the per-chunk address is one add.
+__device__ __forceinline__ const uint32_t* listRowAt(const uint4* row0, uint32_t k)
+{
+    return reinterpret_cast<const uint32_t*>(row0 + (k / kChunk) * 64U);
+}
+
+// The T at the start of 16-byte record j of a particle array, addressed by a
+// 32-bit byte offset from the array base (a scalar base plus a 32-bit vector
+// offset: one shift per gather instead of 64-bit address arithmetic). The
+// engine bounds the particle pool below 2^28 records, so j * 16 fits.
+template <class T>
+__device__ __forceinline__ const T& recAt(const F4* base, uint32_t j)
+{
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (j << 4U));
+}
+
 __device__ __forceinline__ float waveMinF(float v)
 {
     for (int off = 32; off > 0; off >>= 1)
@@ -895,6 +912,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
         const float m0 = sc.m0;
         const float c3 = coefOverH3(h0);
         const float y0 = divRecip(h0);
+        const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
         uint32_t e = 0;
         for (; e + UB <= cnt; e += UB)
         {
@@ -903,7 +921,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < UB; u += 4U)
             {
-                const uint4 v = loadChunk(listAt(c, lp, e + u));
+                const uint4 v = loadChunk(listRowAt(row0, e + u));
                 j[u] = v.x;
                 j[u + 1U] = v.y;
                 j[u + 2U] = v.z;
@@ -912,7 +930,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < UB; ++u)
             {
-                o[u] = *reinterpret_cast<const float3*>(c.s0 + j[u]);
+                o[u] = recAt<float3>(c.s0, j[u]);
             }
 #pragma unroll
             for (uint32_t u = 0; u < UB; ++u)
@@ -925,7 +943,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
         }
         for (; e < cnt; ++e)
         {
-            const float3 o = *reinterpret_cast<const float3*>(c.s0 + *listAt(c, lp, e));
+            const float3 o = recAt<float3>(c.s0, listRowAt(row0, e)[e % kChunk]);
             const float dx = o.x - q.x;
             const float dy = o.y - q.y;
             const float dz = o.z - q.z;
@@ -934,6 +952,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
     }
     else
     {
+        const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
         uint32_t e = 0;
         for (; e + GB <= cnt; e += GB)
         {
@@ -943,7 +962,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < GB; u += 4U)
             {
-                const uint4 v = loadChunk(listAt(c, lp, e + u));
+                const uint4 v = loadChunk(listRowAt(row0, e + u));
                 j[u] = v.x;
                 j[u + 1U] = v.y;
                 j[u + 2U] = v.z;
@@ -952,8 +971,8 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
 #pragma unroll
             for (uint32_t u = 0; u < GB; ++u)
             {
-                o[u] = c.s0[j[u]];
-                hm[u] = c.shm[j[u]];
+                o[u] = recAt<F4>(c.s0, j[u]);
+                hm[u] = recAt<F4>(c.shm, j[u]);
             }
 #pragma unroll
             for (uint32_t u = 0; u < GB; ++u)
@@ -967,9 +986,9 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
         }
         for (; e < cnt; ++e)
         {
-            const uint32_t jj = *listAt(c, lp, e);
-            const F4 o = c.s0[jj];
-            const F4 hm = c.shm[jj];
+            const uint32_t jj = listRowAt(row0, e)[e % kChunk];
+            const F4 o = recAt<F4>(c.s0, jj);
+            const F4 hm = recAt<F4>(c.shm, jj);
             const float dx = o.x - q.x;
             const float dy = o.y - q.y;
             const float dz = o.z - q.z;
@@ -1406,6 +1425,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     const float wd0 = kUniform ? dcoefOverH4(h0) : 0.0F;
     const float y0 = kSharedH ? divRecip(h0) : 0.0F;
     constexpr uint32_t B = kUniform ? kForceUniformBatch : kForceBatch;
+    const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
     uint32_t e = 0;
     for (; e + B <= cnt; e += B)
     {
@@ -1416,7 +1436,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; u += 4U)
         {
-            const uint4 v = loadChunk(listAt(c, lp, e + u));
+            const uint4 v = loadChunk(listRowAt(row0, e + u));
             j[u] = v.x;
             j[u + 1U] = v.y;
             j[u + 2U] = v.z;
@@ -1425,11 +1445,11 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u)
         {
-            fa[u] = c.fa[j[u]];
-            fb[u] = c.fb[j[u]];
+            fa[u] = recAt<F4>(c.fa, j[u]);
+            fb[u] = recAt<F4>(c.fb, j[u]);
             if (!kUniform)
             {
-                hm[u] = c.shm[j[u]];
+                hm[u] = recAt<F4>(c.shm, j[u]);
             }
         }
 #pragma unroll
@@ -1442,19 +1462,19 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     }
     for (; e < cnt; ++e)
     {
-        const uint32_t jj = *listAt(c, lp, e);
+        const uint32_t jj = listRowAt(row0, e)[e % kChunk];
         if (jj != t)
         {
             if (kUniform)
             {
-                forcePair<kSharedH, kLam>(c.fa[jj], c.fb[jj], m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
+                forcePair<kSharedH, kLam>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
                                           nuLam, A,
                                     y0);
             }
             else
             {
-                const F4 h1 = c.shm[jj];
-                forcePair<false, kLam>(c.fa[jj], c.fb[jj], h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
+                const F4 h1 = recAt<F4>(c.shm, jj);
+                forcePair<false, kLam>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
                                        doST, nuLam, A);
             }
         }

@@ -202,7 +202,8 @@ int asph_boundary_generate_device(const asph_domain* domain, float particle_spac
    refinement->enabled; it then uses refinement->max_particle_count as the
    fluid capacity, as the reference does (AdaptiveSphSimulation.cu:37).
    positions: host array of count records. boundary: host array of
-   boundary_count records (may be NULL when boundary_count == 0). */
+   boundary_count records (may be NULL when boundary_count == 0). Fluid
+   capacities above 2^28 - 64 particles are refused (ASPH_ERR_CAPACITY). */
 int asph_create(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
                 const asph_boundary_particle* boundary, uint32_t boundary_count,
                 const asph_refinement_parameters* refinement,

@@ -20,7 +20,7 @@ extern "C" {
 #define ASPH_BATCH_ADAPTIVE 2u /* allocate refinement scratch (adaptive scenes) */
 
 typedef struct asph_batch_config {
-    uint32_t max_particles; /* fluid particle pool (sum of scene capacities) */
+    uint32_t max_particles; /* fluid particle pool (sum of scene capacities), at most 2^28 - 64 */
     uint32_t max_boundary;  /* boundary particle pool */
     uint32_t max_cells;     /* grid cell pool (sum over scenes of cells + 1) */
     uint32_t max_scenes;    /* scene table size */

@@ -102,6 +102,9 @@ def test_validation_before_device_calls():
     rc = lib.asph_create(C.byref(s.params), s.positions.ctypes.data, s.count, None, 0, None, s.count - 1, 1,
                          C.byref(h))
     assert rc == 2 and b"capacity" in lib.asph_last_error()
+    rc = lib.asph_create(C.byref(s.params), s.positions.ctypes.data, s.count, None, 0, None, (1 << 28) - 63, 1,
+                         C.byref(h))
+    assert rc == 2 and b"2^28" in lib.asph_last_error()
     tiny = abi.Parameters.from_buffer_copy(s.params)
     tiny.base_smoothing_radius = 1e-6
     rc = lib.asph_create(C.byref(tiny), s.positions.ctypes.data, s.count, None, 0, None, s.count, 1, C.byref(h))
