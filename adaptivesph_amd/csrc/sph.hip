@@ -1157,7 +1157,21 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
     // quotients (three over d, one over hj) through reciprocals
     const float lo = fminf(fminf(fabsf(ox), fabsf(oy)), fabsf(oz));
     const float hi = fmaxf(fmaxf(fabsf(ox), fabsf(oy)), fabsf(oz));
-    if ((lo >= kDivLo && hi <= 0x1p39F) || !use)
+    bool fast = (lo >= kDivLo && hi <= 0x1p39F) || !use;
+    if (!fast)
+    {
+        // a component of exactly +-0 (particles on one lattice line): its
+        // quotient through the reciprocal is a zero whose sign may differ
+        // from the division's, and a zero of either sign leaves the
+        // accumulators unchanged (they start at +0, and neither a - (+-0) nor
+        // a + (+-0) ever yields -0 from them), so such a pair takes the fast
+        // path as well when its other components are in range and d2 > 0
+        const bool okx = ox == 0.0F || fabsf(ox) >= kDivLo;
+        const bool oky = oy == 0.0F || fabsf(oy) >= kDivLo;
+        const bool okz = oz == 0.0F || fabsf(oz) >= kDivLo;
+        fast = okx && oky && okz && hi <= 0x1p39F && d2 >= 0x1p-80F;
+    }
+    if (fast)
     {
         d = sqrtRanged(d2);
         const float yd = divRecip(d);

@@ -51,6 +51,14 @@ def make_workload(name: str, merged_h: str = "kernel"):
 
     if name == "cfg2":
         return [scene.cfg2()], "dam_break_1M"
+    if name == "cfg2_lattice":
+        # the same dam break on an unjittered lattice (as the reference
+        # application lays out its particles): many pairs share a coordinate
+        import dataclasses
+
+        s = scene.cfg2()
+        pos = scene.lattice(s.meta["n"], s.meta["dx"], (0.0, 0.0, 0.0), s.meta["seed"], jitter=0.0)
+        return [dataclasses.replace(s, name="dam_break_1M_lattice", positions=pos)], "dam_break_1M_lattice"
     if name == "cfg1":
         return [scene.cfg1()], "dam_break_31k"
     if name == "adaptive":
@@ -382,7 +390,7 @@ def main() -> int:
     if args.list_capacity == 0:
         # the library's defaults: refined regions hold up to ~13x denser
         # daughters, and particles there have several hundred neighbours
-        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "copy", "walls") else 640
+        args.list_capacity = 128 if args.workload in ("cfg1", "cfg2", "cfg2_lattice", "copy", "walls") else 640
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); this build steps scenes on one "
               "GPU and leaves multi-GPU sharding out", file=sys.stderr)

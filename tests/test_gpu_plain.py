@@ -4,6 +4,8 @@ Cell keys, sorted indices, cell start/end (with the reference's slot-0
 omission), neighbour sets, densities, accelerations, velocities and positions
 must equal the oracle's bits, on the first step and over whole trajectories.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -49,6 +51,24 @@ def test_cfg1_200_steps_bitexact():
     b = ref.density_info(0.1)
     for f, _ in abi.DensityInfo._fields_:
         assert getattr(a, f) == getattr(b, f), f
+
+
+def test_exact_lattice_bitexact():
+    """An unjittered lattice (as the reference application's particle grid
+    is): pairs on one lattice line have offset components of exactly 0, which
+    the force loop takes through the shared reciprocal."""
+    n = (16, 20, 12)
+    s = scene.dam_break(n, 0.01, (0.5, 0.4, 0.3), seed=1, steps=60, name="exact_lattice")
+    s = dataclasses.replace(s, positions=scene.lattice(n, 0.01, (0.0, 0.0, 0.0), 1, jitter=0.0))
+    assert [len(np.unique(s.positions[:, k])) for k in range(3)] == list(n)
+    sim, ref = _pair(s)
+    for step in range(1, s.steps + 1):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        if step in (1, 10, 60):
+            assert_bitexact(sim.fluid(), ref.fluid(), f"exact lattice step {step}")
+    # the lattice lines persist: particles still share exact coordinates
+    assert len(np.unique(sim.fluid()["positions"][:, 0])) < s.count
 
 
 def test_boundary_terms_active():
