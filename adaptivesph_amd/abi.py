@@ -21,6 +21,7 @@ ASPH_ERR_CAPACITY = 2
 ASPH_ERR_DEVICE = 3
 ASPH_ERR_STATE = 4
 ASPH_FLAG_FIX_CELL_START = 1
+MAX_PARTICLE_CAPACITY = (1 << 28) - 64  # ASPH_MAX_PARTICLE_CAPACITY
 FLAG_KERNEL_MERGED_H = 2
 
 CRITERION = {"velocity": 0, "interface": 1, "vorticity": 2, "zone": 3}

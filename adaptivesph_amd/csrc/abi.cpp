@@ -195,7 +195,7 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
         asph::EngineConfig cfg;
         cfg.particleCapacity = cap;
         cfg.boundaryCapacity = max_boundary_capacity;
-        cfg.cellCapacity = g.cells() + 1U;
+        cfg.cellCapacity = static_cast<uint32_t>(g.cells() + 1U);
         cfg.sceneSlots = 1;
         cfg.listCap = asph::defaultListCap(adaptive);
         if (adaptive)
@@ -566,7 +566,7 @@ int asph_grid_cells(const asph_domain* domain, float smoothing_radius, uint32_t*
     return guard([&] {
         need(domain, "domain");
         need(cells, "cells");
-        *cells = asph::makeGrid(*domain, smoothing_radius).cells();
+        *cells = static_cast<uint32_t>(asph::makeGrid(*domain, smoothing_radius).cells());
     });
 }
 

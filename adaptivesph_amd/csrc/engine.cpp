@@ -54,6 +54,33 @@ uint64_t RangeAllocator::alloc(uint64_t n)
     return UINT64_MAX;
 }
 
+bool RangeAllocator::claim(uint64_t off, uint64_t n)
+{
+    if (n == 0)
+    {
+        return true;
+    }
+    for (size_t k = 0; k < free_.size(); ++k)
+    {
+        const Range r = free_[k];
+        if (r.off <= off && off + n <= r.off + r.n)
+        {
+            free_.erase(free_.begin() + static_cast<long>(k));
+            if (off + n < r.off + r.n)
+            {
+                free_.insert(free_.begin() + static_cast<long>(k), Range {off + n, r.off + r.n - (off + n)});
+            }
+            if (r.off < off)
+            {
+                free_.insert(free_.begin() + static_cast<long>(k), Range {r.off, off - r.off});
+            }
+            used_ += n;
+            return true;
+        }
+    }
+    return false;
+}
+
 void RangeAllocator::grow(uint64_t capacity)
 {
     if (capacity <= cap_)
@@ -101,10 +128,27 @@ GridGeom makeGrid(const asph_domain& d, float h0)
     const float sx = d.max.x - d.min.x;
     const float sy = d.max.y - d.min.y;
     const float sz = d.max.z - d.min.z;
+    const float n[3] = {std::ceil(sx / cellWidth), std::ceil(sy / cellWidth), std::ceil(sz / cellWidth)};
+    double cells = 1.0;
+    for (const float v : n)
+    {
+        // also catches NaN (an empty axis, a non-positive or non-finite h0)
+        if (!(v >= 1.0F) || !(v <= static_cast<float>(kMaxGridCells)))
+        {
+            throw Error(ASPH_ERR_INVALID, "neighbour grid would be empty or need more than 1e9 cells (domain "
+                                          "empty on an axis, or smoothing radius too small for the domain)");
+        }
+        cells *= static_cast<double>(v);
+    }
+    if (cells > static_cast<double>(kMaxGridCells))
+    {
+        throw Error(ASPH_ERR_INVALID, "neighbour grid would need more than 1e9 cells (smoothing radius too small "
+                                      "for the domain)");
+    }
     GridGeom g;
-    g.gx = static_cast<int>(std::ceil(sx / cellWidth));
-    g.gy = static_cast<int>(std::ceil(sy / cellWidth));
-    g.gz = static_cast<int>(std::ceil(sz / cellWidth));
+    g.gx = static_cast<int>(n[0]);
+    g.gy = static_cast<int>(n[1]);
+    g.gz = static_cast<int>(n[2]);
     g.cs[0] = sx / static_cast<float>(g.gx);
     g.cs[1] = sy / static_cast<float>(g.gy);
     g.cs[2] = sz / static_cast<float>(g.gz);
@@ -181,14 +225,7 @@ void validateParams(const asph_parameters& p)
     {
         bad("threads_per_block must be positive");
     }
-    const float cellWidth = (2.0F * p.base_smoothing_radius) * 2.0F;
-    const double cells = std::ceil(static_cast<double>(p.domain.max.x - p.domain.min.x) / cellWidth) *
-                         std::ceil(static_cast<double>(p.domain.max.y - p.domain.min.y) / cellWidth) *
-                         std::ceil(static_cast<double>(p.domain.max.z - p.domain.min.z) / cellWidth);
-    if (!(cells >= 1.0) || cells > 1.0e9)
-    {
-        bad("neighbour grid would need more than 1e9 cells (smoothing radius too small for the domain)");
-    }
+    (void)makeGrid(p.domain, p.base_smoothing_radius);  // throws for grids beyond kMaxGridCells
 }
 
 uint32_t listCapacityRounded(uint32_t listCap)
@@ -959,6 +996,7 @@ void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacin
         throw Error(ASPH_ERR_INVALID, "particle spacing must be positive");
     }
     HostScene& hs0 = scenes_.at(id);
+    (void)makeGrid(dom, hs0.dev.h0);  // refuse a grid beyond the limits before building walls
     const WallFaces wf = wallFaces(dom, spacing, hs0.dev.h0);
     prepareDomain(id, dom, wf.count);
     HostScene& hs = scenes_.at(id);
@@ -992,29 +1030,71 @@ void Engine::prepareDomain(uint32_t id, const asph_domain& dom, uint32_t nb)
     {
         throw Error(ASPH_ERR_INVALID, "domain.max must exceed domain.min on every axis");
     }
-    synchronize();
+    // the same grid limits as at creation, before anything changes
     const GridGeom g = makeGrid(dom, hs.dev.h0);
     const uint64_t cells = g.cells();
-    cAlloc_.release(hs.dev.cellBase, hs.cellRegion);
-    bAlloc_.release(hs.dev.bOff, hs.dev.bCount);
+    synchronize();
     // NeighborGrid::updateBoundarySize reallocates the cell arrays for the new
-    // domain (NeighborGrid.cu:51-85); the pools grow when the new grid or wall
-    // lattice does not fit.
+    // domain (NeighborGrid.cu:51-85). The new ranges are taken first fit with
+    // the scene's old ranges counted as free. When one does not fit, the old
+    // ranges are taken back (nothing else was allocated meanwhile) and each new
+    // range is taken with its pool grown as needed (pool contents move, the
+    // old ranges stay valid); the old ranges are released only once both new
+    // ones are held, so a failure (ASPH_ERR_CAPACITY, a failed hipMalloc)
+    // leaves the scene as it was.
+    const uint64_t oldC = hs.dev.cellBase;
+    const uint64_t oldCn = hs.cellRegion;
+    const uint64_t oldB = hs.dev.bOff;
+    const uint64_t oldBn = hs.dev.bCount;
+    cAlloc_.release(oldC, oldCn);
+    bAlloc_.release(oldB, oldBn);
     uint64_t cBase = cAlloc_.alloc(cells + 1);
-    if (cBase == UINT64_MAX)
-    {
-        growCells(cAlloc_.capacity() + cells + 1);
-        cBase = cAlloc_.alloc(cells + 1);
-    }
     uint64_t bOff = bAlloc_.alloc(nb);
-    if (bOff == UINT64_MAX)
-    {
-        growBoundary(bAlloc_.capacity() + nb);
-        bOff = bAlloc_.alloc(nb);
-    }
     if (cBase == UINT64_MAX || bOff == UINT64_MAX)
     {
-        throw Error(ASPH_ERR_CAPACITY, "pool exhausted while updating the domain");
+        if (cBase != UINT64_MAX)
+        {
+            cAlloc_.release(cBase, cells + 1);
+        }
+        if (bOff != UINT64_MAX)
+        {
+            bAlloc_.release(bOff, nb);
+        }
+        if (!cAlloc_.claim(oldC, oldCn) || !bAlloc_.claim(oldB, oldBn))
+        {
+            throw Error(ASPH_ERR_STATE, "scene ranges lost while updating the domain");
+        }
+        cBase = cAlloc_.alloc(cells + 1);
+        if (cBase == UINT64_MAX)
+        {
+            growCells(cAlloc_.capacity() + cells + 1);
+            cBase = cAlloc_.alloc(cells + 1);
+        }
+        if (cBase == UINT64_MAX)
+        {
+            throw Error(ASPH_ERR_CAPACITY, "cell pool exhausted while updating the domain");
+        }
+        bOff = bAlloc_.alloc(nb);
+        if (bOff == UINT64_MAX)
+        {
+            try
+            {
+                growBoundary(bAlloc_.capacity() + nb);
+            }
+            catch (...)
+            {
+                cAlloc_.release(cBase, cells + 1);
+                throw;
+            }
+            bOff = bAlloc_.alloc(nb);
+        }
+        if (bOff == UINT64_MAX)
+        {
+            cAlloc_.release(cBase, cells + 1);
+            throw Error(ASPH_ERR_CAPACITY, "boundary pool exhausted while updating the domain");
+        }
+        cAlloc_.release(oldC, oldCn);
+        bAlloc_.release(oldB, oldBn);
     }
     SceneDev& d = hs.dev;
     d.dmin[0] = dom.min.x;

@@ -33,6 +33,8 @@ public:
     explicit RangeAllocator(uint64_t capacity = 0);
     // Returns the offset of a free range of n elements, or UINT64_MAX.
     uint64_t alloc(uint64_t n);
+    // Takes the free range [off, off + n) (false if any of it is in use).
+    bool claim(uint64_t off, uint64_t n);
     void release(uint64_t off, uint64_t n);
     void grow(uint64_t capacity);  // appends [capacity(), capacity) as free
     uint64_t capacity() const { return cap_; }
@@ -53,8 +55,12 @@ struct GridGeom
 {
     int gx = 0, gy = 0, gz = 0;
     float cs[3] = {0.F, 0.F, 0.F};
-    uint32_t cells() const { return static_cast<uint32_t>(gx) * static_cast<uint32_t>(gy) * static_cast<uint32_t>(gz); }
+    uint64_t cells() const { return static_cast<uint64_t>(gx) * static_cast<uint64_t>(gy) * static_cast<uint64_t>(gz); }
 };
+// Largest neighbour grid a scene may have (cells).
+constexpr uint64_t kMaxGridCells = 1000000000ULL;
+// Throws Error(ASPH_ERR_INVALID) when the domain is empty on an axis or the
+// grid would have more than kMaxGridCells cells.
 GridGeom makeGrid(const asph_domain& d, float h0);
 // Throws Error(ASPH_ERR_INVALID) for parameters the pipeline cannot run.
 void validateParams(const asph_parameters& p);
@@ -64,7 +70,8 @@ void validateParams(const asph_parameters& p);
 // volume, and refined regions hold several hundred neighbours per particle.
 // Particles beyond the capacity take the slower walking overflow kernels.
 // Largest particle pool: record byte offsets (j * 16) must fit in 32 bits.
-constexpr uint32_t kMaxParticleCapacity = (1U << 28) - 64U;
+constexpr uint32_t kMaxParticleCapacity = ASPH_MAX_PARTICLE_CAPACITY;
+static_assert(kMaxParticleCapacity == (1U << 28) - 64U, "record offsets j * 16 fit 32 bits");
 constexpr uint32_t kDefaultListCapPlain = 128;
 constexpr uint32_t kDefaultListCapAdaptive = 640;
 

@@ -74,10 +74,12 @@ def unit_bytes(list_capacity: int = 0, adaptive: bool = True) -> tuple[int, int,
 def pools_for_budget(budget_bytes: int, list_capacity: int = 0, adaptive: bool = True,
                      boundary_ratio: float = 0.5, cell_ratio: float = 0.25) -> tuple[int, int, int]:
     """Pool sizes (fluid slots, boundary slots, cells) whose device footprint
-    fits `budget_bytes`, with boundary and cell pools sized per fluid slot."""
+    fits `budget_bytes`, with boundary and cell pools sized per fluid slot;
+    the fluid pool is capped at the library's particle capacity limit."""
     fp, fb, fc, _ = unit_bytes(list_capacity, adaptive)
     per = fp + boundary_ratio * fb + cell_ratio * fc
-    p = int(min((budget_bytes - 2 * fc) // per, 2**32 - 2))  # 2 cells beyond the ratio
+    # 2 cells beyond the ratio; the library refuses fluid pools above its capacity limit
+    p = int(min((budget_bytes - 2 * fc) // per, abi.MAX_PARTICLE_CAPACITY))
     return p, int(p * boundary_ratio), int(p * cell_ratio) + 2
 
 

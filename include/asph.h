@@ -29,6 +29,10 @@ extern "C" {
 #define ASPH_ERR_DEVICE 3    /* a HIP runtime call failed */
 #define ASPH_ERR_STATE 4     /* call not valid in the current state */
 
+/* Largest fluid particle capacity of a simulation or batch pool (2^28 - 64):
+   the neighbour loops address particle records by 32-bit byte offsets. */
+#define ASPH_MAX_PARTICLE_CAPACITY 268435392u
+
 typedef struct asph_vec3 { float x, y, z; } asph_vec3;
 typedef struct asph_vec4 { float x, y, z, w; } asph_vec4;
 

@@ -146,3 +146,21 @@ def test_scene_bytes_monotone():
     b = lib.asph_scene_bytes(2000, 0, 100, 0, 128)
     c = lib.asph_scene_bytes(2000, 0, 100, 1, 128)
     assert 0 < a < b < c
+
+
+def test_grid_limits_of_the_engine():
+    """makeGrid refuses grids beyond 1e9 cells or empty on an axis (the limit
+    that creation and updateDomain share), for asph_grid_cells too."""
+    lib = abi.load_library()
+    n = C.c_uint32(0)
+    d = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(2.0, 1.2, 1.0), 0.01)
+    assert lib.asph_grid_cells(C.byref(d), 0.00507, C.byref(n)) == 0 and n.value == 99 * 60 * 50
+    # 2^31 cells along one axis: the float quotient would not fit an int
+    huge = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(1.0e9, 1.0, 1.0), 0.01)
+    assert lib.asph_grid_cells(C.byref(huge), 0.1, C.byref(n)) == abi.ASPH_ERR_INVALID
+    assert b"1e9" in lib.asph_last_error()
+    # 1,200^3 cells: every axis fits, the product does not
+    cube = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(480.0, 480.0, 480.0), 0.01)
+    assert lib.asph_grid_cells(C.byref(cube), 0.1, C.byref(n)) == abi.ASPH_ERR_INVALID
+    flat = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(1.0, 0.0, 1.0), 0.01)
+    assert lib.asph_grid_cells(C.byref(flat), 0.1, C.byref(n)) == abi.ASPH_ERR_INVALID

@@ -120,3 +120,36 @@ def test_batch_scene_moves_its_tank():
     batch.close()
     alone.close()
     other.close()
+
+
+def test_oversized_domain_refused_and_scene_unchanged():
+    """updateDomain with a domain whose grid would exceed 1e9 cells (or is
+    empty on an axis) is refused with ASPH_ERR_INVALID before anything
+    changes: the simulation keeps its grid and walls and steps on exactly as
+    a simulation that never saw the call."""
+    from adaptivesph_amd.simulation import Simulation
+    from helpers import assert_bitexact
+
+    s = scene.dam_break((8, 10, 6), 0.01, (0.3, 0.2, 0.2), seed=4, steps=4)
+    a = Simulation.from_scene(s)
+    b = Simulation.from_scene(s)
+    try:
+        a.update(s.dt)
+        b.update(s.dt)
+        huge = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(480.0, 480.0, 480.0), 0.01)
+        for call in (lambda: a.update_domain(huge, s.boundary),
+                     lambda: a.update_domain_walls(huge, scene.boundary_spacing(s.params)),
+                     lambda: a.update_domain(abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(0.3, 0.0, 0.2), 0.01),
+                                             s.boundary)):
+            with pytest.raises(abi.AsphError) as e:
+                call()
+            assert e.value.status == abi.ASPH_ERR_INVALID
+        assert a.get_boundary_particles_count() == b.get_boundary_particles_count() == len(s.boundary)
+        assert a.grid_dims() == b.grid_dims()
+        for _ in range(s.steps):
+            a.update(s.dt)
+            b.update(s.dt)
+        assert_bitexact(a.fluid(), b.fluid(), "after the refused updateDomain")
+    finally:
+        a.close()
+        b.close()

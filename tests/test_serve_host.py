@@ -169,3 +169,14 @@ def test_pools_fit_budget():
     assert p * fp + b * fb + c * fc <= budget
     assert p * fp + b * fb + c * fc > 0.99 * budget
     assert serve.mean_log_uniform(10_000, 500_000) == pytest.approx(125_253, rel=1e-4)
+
+
+def test_pools_capped_at_the_library_capacity_limit():
+    """A budget larger than the library's fluid pool limit (2^28 - 64 slots,
+    about 230 GB at list capacity 128) sizes the pool to the limit instead of
+    asking for a pool the library refuses."""
+    for lc in (128, 640):
+        p, b, c = serve.pools_for_budget(10**12, list_capacity=lc)
+        assert p <= abi.MAX_PARTICLE_CAPACITY
+    p, _, _ = serve.pools_for_budget(10**12, list_capacity=128)
+    assert p == abi.MAX_PARTICLE_CAPACITY
