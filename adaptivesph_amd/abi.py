@@ -22,6 +22,8 @@ ASPH_ERR_DEVICE = 3
 ASPH_ERR_STATE = 4
 ASPH_FLAG_FIX_CELL_START = 1
 MAX_PARTICLE_CAPACITY = (1 << 28) - 64  # ASPH_MAX_PARTICLE_CAPACITY
+NOALLOC_MAX_WALLS = 1 << 20  # ASPH_NOALLOC_MAX_WALLS
+NOALLOC_MAX_BUCKETS = 1 << 22  # ASPH_NOALLOC_MAX_BUCKETS
 FLAG_KERNEL_MERGED_H = 2
 
 CRITERION = {"velocity": 0, "interface": 1, "vorticity": 2, "zone": 3}
@@ -234,6 +236,7 @@ def _declare(lib: C.CDLL) -> None:
         "asph_default_refinement_parameters": (C.c_int, [P(RefinementParameters)]),
         "asph_boundary_generate": (C.c_int, [P(Domain), f32, f32, f32, vp, P(u32)]),
         "asph_boundary_generate_device": (C.c_int, [P(Domain), f32, f32, f32, vp, P(u32)]),
+        "asph_boundary_generate_device_noalloc": (C.c_int, [P(Domain), f32, f32, f32, vp, P(u32), vp]),
         "asph_create": (
             C.c_int,
             [P(Parameters), vp, u32, vp, u32, P(RefinementParameters), u32, u32, P(vp)],

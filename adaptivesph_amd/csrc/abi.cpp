@@ -167,6 +167,26 @@ int asph_boundary_generate_device(const asph_domain* domain, float particle_spac
     });
 }
 
+int asph_boundary_generate_device_noalloc(const asph_domain* domain, float particle_spacing, float rest_density,
+                                          float smoothing_radius, asph_boundary_particle* out, uint32_t* count,
+                                          void* stream)
+{
+    return guard([&] {
+        need(domain, "domain");
+        need(count, "count");
+        if (!(particle_spacing > 0.0F) || !(smoothing_radius > 0.0F))
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "spacing and smoothing radius must be positive");
+        }
+        const int rc = asph::boundaryGenerateNoAlloc(*domain, particle_spacing, rest_density, smoothing_radius, out,
+                                                     count, static_cast<hipStream_t>(stream));
+        if (rc != ASPH_OK)
+        {
+            throw asph::Error(rc, "output array too small");
+        }
+    });
+}
+
 int asph_create(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
                 const asph_boundary_particle* boundary, uint32_t boundary_count,
                 const asph_refinement_parameters* refinement, uint32_t max_fluid_capacity,

@@ -91,37 +91,70 @@ std::vector<P4> wallLattice(const asph_domain& d, float spacing)
     return out;
 }
 
-WallFaces wallFaces(const asph_domain& d, float spacing, float h)
+int wallSequences(const asph_domain& d, float spacing, uint64_t maxTotal, WallSequences* out)
 {
-    // the loops of wallLattice: X, Y, Z from the min corner (the faces that
-    // start one step in begin at the sequence's second value: the same FP32
-    // additions)
+    // the loops of wallLattice: X, Y, Z from the min corner, counted without
+    // storing them (the values are the same FP32 additions wherever they run)
     const float off = 0.05F;
     const float mn[3] = {d.min.x - off, d.min.y - off, d.min.z - off};
     const float mx[3] = {d.max.x + off, d.max.y + off, d.max.z + off};
-    WallFaces wf;
-    uint32_t seqOff[3];
-    uint32_t seqLen[3];
+    uint64_t total = 0;
     for (int a = 0; a < 3; ++a)
     {
-        seqOff[a] = static_cast<uint32_t>(wf.coords.size());
-        for (float v = mn[a]; v < mx[a]; v += spacing)
+        out->mn[a] = mn[a];
+        out->mx[a] = mx[a];
+        uint32_t len = 0;
+        float second = mn[a];
+        float last = mn[a];
+        for (float v = mn[a]; v < mx[a];)
         {
-            wf.coords.push_back(v);
+            if (len == 1U)
+            {
+                second = v;
+            }
+            last = v;
+            ++len;
+            if (++total > maxTotal)
+            {
+                return kWallSeqTooLong;
+            }
+            const float next = v + spacing;
+            if (!(next > v))
+            {
+                return kWallSeqStalls;  // spacing below half an ulp of v (or not a positive number)
+            }
+            v = next;
         }
-        seqLen[a] = static_cast<uint32_t>(wf.coords.size()) - seqOff[a];
+        out->len[a] = len;
+        out->second[a] = second;
+        out->last[a] = last;
     }
-    auto full = [&](int a) { return std::pair<uint32_t, uint32_t>(seqOff[a], seqLen[a]); };
+    return kWallSeqOk;
+}
+
+void wallFaceTable(const WallSequences& sq, float h, WallFaces* wf)
+{
+    // coordinate sequences back to back: X at 0, then Y, then Z; the faces
+    // that start one step in begin at the sequence's second value
+    const uint32_t seqOff[3] = {0U, sq.len[0], sq.len[0] + sq.len[1]};
+    struct Range
+    {
+        uint32_t off, n;
+        float lo, hi;  // smallest and largest value of the range (the sequences increase)
+    };
+    auto full = [&](int a) { return Range {seqOff[a], sq.len[a], sq.mn[a], sq.last[a]}; };
     auto tail = [&](int a) {
-        return seqLen[a] == 0U ? std::pair<uint32_t, uint32_t>(seqOff[a], 0U)
-                               : std::pair<uint32_t, uint32_t>(seqOff[a] + 1U, seqLen[a] - 1U);
+        return sq.len[a] == 0U ? Range {seqOff[a], 0U, 0.F, 0.F}
+                               : Range {seqOff[a] + 1U, sq.len[a] - 1U, sq.second[a], sq.last[a]};
     };
     struct Spec
     {
-        std::pair<uint32_t, uint32_t> A, B;
+        Range A, B;
         int axisA, axisB, axisFixed;
         float fixed;
     };
+    const float* mn = sq.mn;
+    const float* mx = sq.mx;
     const Spec spec[6] = {
         {full(0), full(2), 0, 2, 1, mn[1]}, {full(0), full(2), 0, 2, 1, mx[1]},
         {tail(1), full(2), 1, 2, 0, mn[0]}, {tail(1), full(2), 1, 2, 0, mx[0]},
@@ -133,16 +166,16 @@ WallFaces wallFaces(const asph_domain& d, float spacing, float h)
     for (int f = 0; f < 6; ++f)
     {
         const Spec& sp = spec[f];
-        WallFace& w = wf.face[f];
-        w.offA = sp.A.first;
-        w.nA = sp.A.second;
-        w.offB = sp.B.first;
-        w.nB = sp.B.second;
+        WallFace& w = wf->face[f];
+        w.offA = sp.A.off;
+        w.nA = sp.A.n;
+        w.offB = sp.B.off;
+        w.nB = sp.B.n;
         w.axisA = sp.axisA;
         w.axisB = sp.axisB;
         w.axisFixed = sp.axisFixed;
         w.fixed = sp.fixed;
-        w.start = static_cast<uint32_t>(start);
+        w.start = static_cast<uint32_t>(std::min<uint64_t>(start, UINT32_MAX));
         const uint64_t n = static_cast<uint64_t>(w.nA) * w.nB;
         start += n;
         if (n == 0)
@@ -150,15 +183,10 @@ WallFaces wallFaces(const asph_domain& d, float spacing, float h)
             continue;
         }
         // the face's extent (the bucket grid spans the particles themselves)
-        auto span = [&](uint32_t o, uint32_t len, int axis) {
-            for (uint32_t k = o; k < o + len; ++k)
-            {
-                lo[axis] = std::min(lo[axis], static_cast<double>(wf.coords[k]));
-                hi[axis] = std::max(hi[axis], static_cast<double>(wf.coords[k]));
-            }
-        };
-        span(w.offA, w.nA, w.axisA);
-        span(w.offB, w.nB, w.axisB);
+        lo[w.axisA] = std::min(lo[w.axisA], static_cast<double>(sp.A.lo));
+        hi[w.axisA] = std::max(hi[w.axisA], static_cast<double>(sp.A.hi));
+        lo[w.axisB] = std::min(lo[w.axisB], static_cast<double>(sp.B.lo));
+        hi[w.axisB] = std::max(hi[w.axisB], static_cast<double>(sp.B.hi));
         lo[w.axisFixed] = std::min(lo[w.axisFixed], static_cast<double>(w.fixed));
         hi[w.axisFixed] = std::max(hi[w.axisFixed], static_cast<double>(w.fixed));
     }
@@ -166,14 +194,35 @@ WallFaces wallFaces(const asph_domain& d, float spacing, float h)
     {
         throw std::runtime_error("too many wall particles");
     }
-    wf.count = static_cast<uint32_t>(start);
-    wf.buckets.w = static_cast<double>(2.0F * h);
+    wf->count = static_cast<uint32_t>(start);
+    wf->buckets.w = static_cast<double>(2.0F * h);
     for (int a = 0; a < 3; ++a)
     {
-        wf.buckets.lo[a] = wf.count == 0 ? 0.0 : lo[a];
-        wf.buckets.dims[a] =
-            wf.count == 0 ? 1 : std::max(1, static_cast<int>(std::floor((hi[a] - lo[a]) / wf.buckets.w)) + 1);
+        wf->buckets.lo[a] = wf->count == 0 ? 0.0 : lo[a];
+        wf->buckets.dims[a] =
+            wf->count == 0 ? 1 : std::max(1, static_cast<int>(std::floor((hi[a] - lo[a]) / wf->buckets.w)) + 1);
     }
+}
+
+WallFaces wallFaces(const asph_domain& d, float spacing, float h)
+{
+    WallSequences sq;
+    if (wallSequences(d, spacing, 1ULL << 31, &sq) != kWallSeqOk)
+    {
+        throw std::runtime_error("wall coordinate sequences do not advance or are too long");
+    }
+    WallFaces wf;
+    wf.coords.reserve(static_cast<size_t>(sq.len[0]) + sq.len[1] + sq.len[2]);
+    for (int a = 0; a < 3; ++a)
+    {
+        float v = sq.mn[a];
+        for (uint32_t k = 0; k < sq.len[a]; ++k)
+        {
+            wf.coords.push_back(v);
+            v = v + spacing;
+        }
+    }
+    wallFaceTable(sq, h, &wf);
     return wf;
 }
 
@@ -307,6 +356,12 @@ void wallPsi(const std::vector<P4>& pts, float h, float rho0, std::vector<float>
 int boundaryGenerate(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
                      uint32_t* count, bool onDevice)
 {
+    WallSequences sq;
+    if (wallSequences(d, spacing, 1ULL << 31, &sq) != kWallSeqOk)
+    {
+        throw std::runtime_error("wall coordinate sequences do not advance (spacing too small for the domain's "
+                                 "coordinates) or are too long");
+    }
     const std::vector<P4> pts = wallLattice(d, spacing);
     if (out == nullptr)
     {

@@ -11,10 +11,12 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <vector>
 
 #include "device_common.hpp"
+#include "engine.hpp"
 #include "kernels.hpp"
 #include "walls.hpp"
 
@@ -443,6 +445,69 @@ struct DevBuf
     DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Device buffers of one psi computation over n wall particles.
+struct PsiScratch
+{
+    uint32_t* keysA;
+    uint32_t* valsA;
+    uint32_t* keysB;
+    uint32_t* valsB;
+    uint32_t* start;  // [buckets + 1]
+    uint32_t* hist;   // [radixHistWords(n, radixBits)]
+    uint32_t* digitTotals;  // [1 << radixBits]
+};
+
+struct PsiPlan
+{
+    BucketGrid g;
+    uint32_t buckets;
+    int bits;       // key width of the bucket keys
+    int radixBits;  // digit width of their sort
+};
+
+PsiPlan psiPlan(const WallBuckets& wb)
+{
+    PsiPlan pl;
+    for (int a = 0; a < 3; ++a)
+    {
+        pl.g.lo[a] = wb.lo[a];
+        pl.g.dims[a] = wb.dims[a];
+    }
+    pl.g.w = wb.w;
+    const uint64_t buckets64 = static_cast<uint64_t>(pl.g.dims[0]) * static_cast<uint64_t>(pl.g.dims[1]) *
+                               static_cast<uint64_t>(pl.g.dims[2]);
+    if (buckets64 >= (1ULL << 31))
+    {
+        throw std::runtime_error("wall bucket grid too large");
+    }
+    pl.buckets = static_cast<uint32_t>(buckets64);
+    int bits = 1;
+    while (bits < 32 && (1ULL << bits) < buckets64)
+    {
+        ++bits;
+    }
+    pl.bits = bits;
+    pl.radixBits = radixDigitBits(0, bits);
+    return pl;
+}
+
+// Keys, stable sort, bucket ranges and psi, enqueued on stream s (every
+// scratch word that is read is written first: nothing needs clearing).
+void wallPsiEnqueue(const F4* pos, uint32_t n, const PsiPlan& pl, float h, float rho0, float* psi,
+                    const PsiScratch& sc, hipStream_t s)
+{
+    const uint32_t blocks = std::min<uint32_t>((n + 255U) / 256U, 4096U);
+    hipLaunchKernelGGL(kWallKeys, dim3(blocks), dim3(256), 0, s, pos, n, pl.g, sc.keysA, sc.valsA);
+    const int inB = radixSortU32(sc.keysA, sc.valsA, sc.keysB, sc.valsB, nullptr, n, pl.bits, pl.radixBits,
+                                 sc.hist, sc.digitTotals, s);
+    const uint32_t* sk = inB != 0 ? sc.keysB : sc.keysA;
+    const uint32_t* sv = inB != 0 ? sc.valsB : sc.valsA;
+    const uint32_t bblocks = std::min<uint32_t>((pl.buckets + 256U) / 256U, 4096U);
+    hipLaunchKernelGGL(kWallBucketStarts, dim3(bblocks), dim3(256), 0, s, sk, n, pl.buckets, sc.start);
+    const uint32_t pblocks = std::min<uint32_t>((n + kPsiBlock - 1U) / kPsiBlock, 16384U);
+    hipLaunchKernelGGL(kWallPsi, dim3(pblocks), dim3(kPsiBlock), 0, s, pos, n, pl.g, sv, sc.start, h, rho0, psi);
+}
+
 }  // namespace
 
 void wallPsiOnDevice(const F4* pos, uint32_t n, const WallBuckets& wb, float h, float rho0, float* psi,
@@ -452,40 +517,13 @@ void wallPsiOnDevice(const F4* pos, uint32_t n, const WallBuckets& wb, float h, 
     {
         return;
     }
-    BucketGrid g;
-    for (int a = 0; a < 3; ++a)
-    {
-        g.lo[a] = wb.lo[a];
-        g.dims[a] = wb.dims[a];
-    }
-    g.w = wb.w;
-    const uint64_t buckets64 = static_cast<uint64_t>(g.dims[0]) * static_cast<uint64_t>(g.dims[1]) *
-                               static_cast<uint64_t>(g.dims[2]);
-    if (buckets64 >= (1ULL << 31))
-    {
-        throw std::runtime_error("wall bucket grid too large");
-    }
-    const uint32_t buckets = static_cast<uint32_t>(buckets64);
-    int bits = 1;
-    while (bits < 32 && (1ULL << bits) < buckets64)
-    {
-        ++bits;
-    }
-    const int radixBits = radixDigitBits(0, bits);
+    const PsiPlan pl = psiPlan(wb);
     DevBuf<uint32_t> keysA(n), valsA(n), keysB(n), valsB(n);
-    DevBuf<uint32_t> start(static_cast<size_t>(buckets) + 1U);
-    DevBuf<uint32_t> hist(radixHistWords(n, radixBits));
-    DevBuf<uint32_t> digitTotals(static_cast<size_t>(1U) << radixBits);
-    const uint32_t blocks = std::min<uint32_t>((n + 255U) / 256U, 4096U);
-    hipLaunchKernelGGL(kWallKeys, dim3(blocks), dim3(256), 0, s, pos, n, g, keysA.p, valsA.p);
-    const int inB = radixSortU32(keysA.p, valsA.p, keysB.p, valsB.p, nullptr, n, bits, radixBits, hist.p,
-                                 digitTotals.p, s);
-    const uint32_t* sk = inB != 0 ? keysB.p : keysA.p;
-    const uint32_t* sv = inB != 0 ? valsB.p : valsA.p;
-    const uint32_t bblocks = std::min<uint32_t>((buckets + 256U) / 256U, 4096U);
-    hipLaunchKernelGGL(kWallBucketStarts, dim3(bblocks), dim3(256), 0, s, sk, n, buckets, start.p);
-    const uint32_t pblocks = std::min<uint32_t>((n + kPsiBlock - 1U) / kPsiBlock, 16384U);
-    hipLaunchKernelGGL(kWallPsi, dim3(pblocks), dim3(kPsiBlock), 0, s, pos, n, g, sv, start.p, h, rho0, psi);
+    DevBuf<uint32_t> start(static_cast<size_t>(pl.buckets) + 1U);
+    DevBuf<uint32_t> hist(radixHistWords(n, pl.radixBits));
+    DevBuf<uint32_t> digitTotals(static_cast<size_t>(1U) << pl.radixBits);
+    wallPsiEnqueue(pos, n, pl, h, rho0, psi,
+                   PsiScratch {keysA.p, valsA.p, keysB.p, valsB.p, start.p, hist.p, digitTotals.p}, s);
     hipCheck(hipGetLastError(), "wall psi launch");
     // the temporaries are freed when this returns: finish with them first
     hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -566,6 +604,173 @@ void wallsOnDevice(const WallFaces& wf, const SceneDev& d, const WallTargets& ou
     hipCheck(hipMemcpy(m, meta.p, sizeof(m), hipMemcpyDeviceToHost), "download");
     *lostEnd = m[0];
     *inside = m[1] != 0U;
+}
+
+// ---------------------------------------- walls without device allocation --
+// asph_boundary_generate_device_noalloc works in one fixed array per GPU,
+// `scratch`: part of the library's code object, so each GPU the library runs
+// on holds it from the code object's load until the process ends. Every call
+// on a GPU shares that GPU's array (calls that overlap on one GPU race); every
+// word a call reads is written earlier in the same call, so nothing is
+// cleared between calls.
+namespace
+{
+
+constexpr uint32_t kNoAllocMaxWalls = ASPH_NOALLOC_MAX_WALLS;
+constexpr uint32_t kNoAllocMaxBuckets = ASPH_NOALLOC_MAX_BUCKETS;
+constexpr uint32_t kNoAllocMaxCoords = 1U << 16;  // values of the three coordinate sequences together
+constexpr size_t kNoAllocHistWords = static_cast<size_t>(512) << kMaxRadixBits;
+
+constexpr size_t align256(size_t b) { return (b + 255U) & ~static_cast<size_t>(255U); }
+
+constexpr size_t kOffPos = 0;
+constexpr size_t kOffPsi = kOffPos + align256(sizeof(F4) * kNoAllocMaxWalls);
+constexpr size_t kOffKeysA = kOffPsi + align256(sizeof(float) * kNoAllocMaxWalls);
+constexpr size_t kOffValsA = kOffKeysA + align256(sizeof(uint32_t) * kNoAllocMaxWalls);
+constexpr size_t kOffKeysB = kOffValsA + align256(sizeof(uint32_t) * kNoAllocMaxWalls);
+constexpr size_t kOffValsB = kOffKeysB + align256(sizeof(uint32_t) * kNoAllocMaxWalls);
+constexpr size_t kOffStart = kOffValsB + align256(sizeof(uint32_t) * kNoAllocMaxWalls);
+constexpr size_t kOffHist = kOffStart + align256(sizeof(uint32_t) * (static_cast<size_t>(kNoAllocMaxBuckets) + 1U));
+constexpr size_t kOffDigits = kOffHist + align256(sizeof(uint32_t) * kNoAllocHistWords);
+constexpr size_t kOffCoords = kOffDigits + align256(sizeof(uint32_t) << kMaxRadixBits);
+constexpr size_t kScratchBytes = kOffCoords + align256(sizeof(float) * kNoAllocMaxCoords);
+static_assert(kScratchBytes < (64ULL << 20), "scratch of the no-alloc wall generator stays under 64 MiB");
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace
+{
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+unsigned char* g_scratchPtr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU, looked up once per device
+// (each GPU holds its own copy of the array at its own address).
+unsigned char* scratchOnCurrentDevice()
+{
+    int dev = 0;
+    hipCheck(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices)
+    {
+        throw Error(ASPH_ERR_DEVICE, "device index beyond the scratch table");
+    }
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (g_scratchPtr[dev] == nullptr)
+    {
+        void* p = nullptr;
+        hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratchPtr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratchPtr[dev];
+}
+
+// The coordinate sequences of wallLattice (X, then Y, then Z), one lane per
+// axis: the same FP32 additions as the host loops.
+__global__ __launch_bounds__(64) void kWallCoords(WallSequences sq, float spacing, float* coords)
+{
+    const uint32_t a = threadIdx.x;
+    if (a >= 3U)
+    {
+        return;
+    }
+    const uint32_t off = a == 0U ? 0U : (a == 1U ? sq.len[0] : sq.len[0] + sq.len[1]);
+    float v = sq.mn[a];
+    for (uint32_t k = 0; k < sq.len[a]; ++k)
+    {
+        coords[off + k] = v;
+        v = v + spacing;
+    }
+}
+
+// StaticBoundaryDomain::Particle records (position, psi)
+__global__ __launch_bounds__(256) void kWallPack(const F4* pos, const float* psi, uint32_t n,
+                                                 asph_boundary_particle* out)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    {
+        const F4 p = pos[i];
+        asph_boundary_particle r;
+        r.position = asph_vec4 {p.x, p.y, p.z, p.w};
+        r.psi = psi[i];
+        out[i] = r;
+    }
+}
+
+}  // namespace
+
+int boundaryGenerateNoAlloc(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
+                            uint32_t* count, hipStream_t s)
+{
+    WallSequences sq;
+    const int seq = wallSequences(d, spacing, kNoAllocMaxCoords, &sq);
+    if (seq == kWallSeqTooLong)
+    {
+        throw Error(ASPH_ERR_CAPACITY, "wall coordinate sequences hold more than 65,536 values (the limit of the "
+                                       "allocation-free generator)");
+    }
+    if (seq != kWallSeqOk)
+    {
+        throw Error(ASPH_ERR_INVALID, "wall coordinate sequences do not advance (spacing too small for the "
+                                      "domain's coordinates)");
+    }
+    WallFaces wf;
+    wallFaceTable(sq, h, &wf);
+    if (out == nullptr)
+    {
+        *count = wf.count;
+        return ASPH_OK;
+    }
+    if (*count < wf.count)
+    {
+        return ASPH_ERR_CAPACITY;
+    }
+    if (wf.count > kNoAllocMaxWalls)
+    {
+        throw Error(ASPH_ERR_CAPACITY, "the domain has " + std::to_string(wf.count) +
+                                           " wall particles; the allocation-free generator takes at most " +
+                                           std::to_string(kNoAllocMaxWalls));
+    }
+    const uint32_t n = wf.count;
+    if (n == 0)
+    {
+        *count = 0;
+        return ASPH_OK;
+    }
+    const PsiPlan pl = psiPlan(wf.buckets);
+    if (pl.buckets > kNoAllocMaxBuckets)
+    {
+        throw Error(ASPH_ERR_CAPACITY, "the wall bucket grid has " + std::to_string(pl.buckets) +
+                                           " buckets; the allocation-free generator takes at most " +
+                                           std::to_string(kNoAllocMaxBuckets));
+    }
+    if (radixHistWords(n, pl.radixBits) > kNoAllocHistWords)
+    {
+        throw Error(ASPH_ERR_CAPACITY, "radix histogram beyond the allocation-free generator's scratch");
+    }
+    unsigned char* base = scratchOnCurrentDevice();
+    F4* pos = reinterpret_cast<F4*>(base + kOffPos);
+    float* psi = reinterpret_cast<float*>(base + kOffPsi);
+    float* coords = reinterpret_cast<float*>(base + kOffCoords);
+    const PsiScratch sc {reinterpret_cast<uint32_t*>(base + kOffKeysA), reinterpret_cast<uint32_t*>(base + kOffValsA),
+                         reinterpret_cast<uint32_t*>(base + kOffKeysB), reinterpret_cast<uint32_t*>(base + kOffValsB),
+                         reinterpret_cast<uint32_t*>(base + kOffStart), reinterpret_cast<uint32_t*>(base + kOffHist),
+                         reinterpret_cast<uint32_t*>(base + kOffDigits)};
+    hipLaunchKernelGGL(kWallCoords, dim3(1), dim3(64), 0, s, sq, spacing, coords);
+    FaceSet fs;
+    for (int f = 0; f < 6; ++f)
+    {
+        fs.f[f] = wf.face[f];
+    }
+    const uint32_t blocks = std::max<uint32_t>(std::min<uint32_t>((n + 255U) / 256U, 4096U), 1U);
+    hipLaunchKernelGGL(kWallLattice, dim3(blocks), dim3(256), 0, s, fs, coords, n, pos);
+    wallPsiEnqueue(pos, n, pl, h, rho0, psi, sc, s);
+    hipLaunchKernelGGL(kWallPack, dim3(blocks), dim3(256), 0, s, pos, psi, n, out);
+    hipCheck(hipGetLastError(), "allocation-free wall generator launch");
+    *count = n;
+    return ASPH_OK;
 }
 
 }  // namespace asph

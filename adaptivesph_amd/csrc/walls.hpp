@@ -61,6 +61,32 @@ struct WallFaces
 
 WallFaces wallFaces(const asph_domain& d, float spacing, float h);
 
+// The three coordinate sequences of wallLattice, counted without storing
+// them: min corner mn, loop bound mx, length, second and last value per axis
+// (the sequences increase). Fails when a sequence does not advance
+// (spacing below half an ulp of a coordinate) or the lengths together exceed
+// maxTotal (kWallSeq* codes).
+struct WallSequences
+{
+    float mn[3], mx[3];
+    uint32_t len[3];
+    float second[3], last[3];
+};
+constexpr int kWallSeqOk = 0;
+constexpr int kWallSeqStalls = 1;   // a sequence does not advance
+constexpr int kWallSeqTooLong = 2;  // more than maxTotal values
+int wallSequences(const asph_domain& d, float spacing, uint64_t maxTotal, WallSequences* out);
+// Face table, particle count and bucket grid of wallFaces from the sequences
+// (coords of wf left untouched).
+void wallFaceTable(const WallSequences& sq, float h, WallFaces* wf);
+
+// StaticBoundaryDomain::generate with the lattice, psi and output records all
+// built on the current GPU in the library's fixed scratch array (no device
+// allocation), enqueued on stream s; out is device memory of *count records
+// (NULL: *count receives the wall count only).
+int boundaryGenerateNoAlloc(const asph_domain& d, float spacing, float rho0, float h, asph_boundary_particle* out,
+                            uint32_t* count, hipStream_t s);
+
 // psi of n device-resident wall positions (w ignored) into psi[n] (device);
 // synchronous on stream s.
 struct F4;

@@ -201,6 +201,26 @@ int asph_boundary_generate_device(const asph_domain* domain, float particle_spac
                                   float rest_density, float smoothing_radius,
                                   asph_boundary_particle* out, uint32_t* count);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the records of asph_boundary_generate_device, bit for
+   bit, built with no device allocation and no host-device copy, in a fixed
+   scratch array of about 54 MiB that the library holds on each GPU it runs on
+   until the process ends. out is DEVICE memory of *count records on the
+   current GPU (NULL: *count receives the wall count only, no GPU work).
+   stream is a hipStream_t (NULL: the null stream); the call returns once the
+   work is enqueued, so synchronise the stream before reading out. Limits
+   (ASPH_ERR_CAPACITY beyond them): at most ASPH_NOALLOC_MAX_WALLS wall
+   particles, ASPH_NOALLOC_MAX_BUCKETS buckets of width 2 smoothing_radius over
+   the walls' box and 65,536 values of the three coordinate sequences. Calls
+   share their GPU's scratch array: calls that overlap on one GPU (other
+   streams or threads) race. */
+#define ASPH_NOALLOC_MAX_WALLS 1048576u
+#define ASPH_NOALLOC_MAX_BUCKETS 4194304u
+int asph_boundary_generate_device_noalloc(const asph_domain* domain, float particle_spacing,
+                                          float rest_density, float smoothing_radius,
+                                          asph_boundary_particle* out, uint32_t* count, void* stream);
+
 /* createSimulation (Simulation.cuh:138-145, SimulationFactory.cu:14-40).
    The adaptive variant is chosen when refinement != NULL and
    refinement->enabled; it then uses refinement->max_particle_count as the

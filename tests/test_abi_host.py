@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 from adaptivesph_amd import abi, scene
+from conftest import HAVE_GPU
 from oracle import oracle
 
 ROOT = Path(__file__).resolve().parent.parent
@@ -146,6 +147,42 @@ def test_scene_bytes_monotone():
     b = lib.asph_scene_bytes(2000, 0, 100, 0, 128)
     c = lib.asph_scene_bytes(2000, 0, 100, 1, 128)
     assert 0 < a < b < c
+
+
+def _noalloc_call(domain, spacing, h, out, capacity):
+    lib = abi.load_library()
+    n = C.c_uint32(capacity)
+    rc = lib.asph_boundary_generate_device_noalloc(C.byref(domain), spacing, 1000.0, h, out, C.byref(n), None)
+    return rc, n.value, lib.asph_last_error()
+
+
+def test_noalloc_walls_count_and_limits_without_gpu_work():
+    """The allocation-free wall generator answers a count query (out = NULL)
+    with the host generator's count, and refuses, before any GPU work, domains
+    past its limits: more coordinate values than its scratch holds, more wall
+    particles than ASPH_NOALLOC_MAX_WALLS, and a spacing that cannot advance
+    the coordinate sequences."""
+    lib = abi.load_library()
+    for s in (scene.cfg1(boundary=""), scene.cfg2(boundary="")):
+        spacing = scene.boundary_spacing(s.params)
+        rc, n, _ = _noalloc_call(s.params.domain, spacing, s.params.base_smoothing_radius, None, 0)
+        ref = C.c_uint32(0)
+        assert lib.asph_boundary_generate(C.byref(s.params.domain), spacing, 1000.0, s.params.base_smoothing_radius,
+                                          None, C.byref(ref)) == 0
+        assert rc == 0 and n == ref.value
+    s = scene.cfg2(boundary="")
+    # 1.44M walls at spacing 3 mm: counted, but refused as output (a host
+    # check: no GPU is visible here, and no device buffer is touched)
+    rc, n, _ = _noalloc_call(s.params.domain, 0.003, 0.004, None, 0)
+    assert rc == 0 and n > abi.NOALLOC_MAX_WALLS
+    if not HAVE_GPU:
+        rc, _, msg = _noalloc_call(s.params.domain, 0.003, 0.004, 1, 1 << 21)
+        assert rc == abi.ASPH_ERR_CAPACITY and b"at most 1048576" in msg
+    rc, _, msg = _noalloc_call(s.params.domain, 1e-5, 0.004, None, 0)
+    assert rc == abi.ASPH_ERR_CAPACITY and b"65,536" in msg
+    big = abi.Domain(abi.Vec3(1.0e6, 0.0, 0.0), abi.Vec3(1.0e6 + 1.0, 1.0, 1.0), 0.01)
+    rc, _, msg = _noalloc_call(big, 0.01, 0.004, None, 0)
+    assert rc == abi.ASPH_ERR_INVALID and b"advance" in msg
 
 
 def test_grid_limits_of_the_engine():
