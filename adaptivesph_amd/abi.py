@@ -290,6 +290,7 @@ def _extra_sigs():
         "asph_batch_synchronize": (C.c_int, [vp]),
         "asph_batch_step_timed": (C.c_int, [vp, P(f32)]),
         "asph_batch_scene_count": (C.c_int, [vp, u32, P(u32)]),
+        "asph_batch_average_neighbor_count": (C.c_int, [vp, u32, P(f32)]),
         "asph_batch_copy_scene": (C.c_int, [vp, u32, u32, vp, vp, vp, vp, vp, vp, vp]),
         "asph_batch_set_launch": (C.c_int, [vp, u32, u32, u32, u32]),
         "asph_batch_memory": (C.c_int, [vp, P(u64), P(u64)]),

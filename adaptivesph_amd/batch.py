@@ -106,6 +106,13 @@ class Batch:
         self._check(self._lib.asph_batch_scene_count(self._h, int(sid), C.byref(n)), "count")
         return int(n.value)
 
+    def average_neighbor_count(self, sid: int) -> float:
+        """calculateAverageNeighborCount of scene `sid` (grid of the last step)."""
+        v = C.c_float()
+        self._check(self._lib.asph_batch_average_neighbor_count(self._h, int(sid), C.byref(v)),
+                    "average neighbour count")
+        return float(v.value)
+
     def fluid(self, sid: int, mask: int = abi.FIELD_ALL) -> dict[str, np.ndarray]:
         n = self.count(sid)
         out = {

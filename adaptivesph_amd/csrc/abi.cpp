@@ -496,6 +496,16 @@ int asph_batch_step_particles(asph_batch* batch, uint32_t* count)
     });
 }
 
+int asph_batch_average_neighbor_count(asph_batch* batch, uint32_t scene_id, float* out)
+{
+    return guard([&] {
+        need(out, "out");
+        asph::Engine& e = engineOf(batch);
+        checkScene(e, scene_id);
+        *out = e.averageNeighborCount(scene_id);
+    });
+}
+
 int asph_batch_scene_count(asph_batch* batch, uint32_t scene_id, uint32_t* count)
 {
     return guard([&] {

@@ -6,7 +6,14 @@ lattice, dx = 0.0039 m, 2.0 x 1.2 x 1.0 m box, FP32, no refinement), stepped
 as one scene through the batched engine with the step captured as a hipGraph.
 A step is one full pass of the pipeline (kick, drift, cell keys, sort, cell
 ranges, gather, density, forces, kick) over all particles. The inputs are
-resident in device memory before the timed region. Prints one JSON line.
+resident in device memory before the timed region. Prints one JSON line whose
+value is the 1M dam break's particle-updates/s; the same line carries the
+second half of the headline metric under "batched" (256 scenes stepped
+together: scene-steps/s, p50 / p99 step latency, speed-up over the same scenes
+stepped one after another), per-kernel figures of the neighbour stages
+("kernels": algorithmic bytes and flops against the HBM and FP32 peaks, and
+the stored counter figures) and the CPU baseline (the oracle, median of 3 runs
+of 50 steps on the host's cores).
 
 Other workloads (--workload):
   cfg1      31,250-particle dam break (1.0 x 0.6 x 0.5 m box)
@@ -35,11 +42,24 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector peak
+HEADLINE_METRIC = "particle-updates/s at 1M WCSPH; batched scenes/s + p99 step latency"
 # Algorithmic bytes per particle and launch (minimum compulsory traffic):
 #   density: reads position 16 B, h 4 B, mass 4 B; writes density 4 B
 #   force (+ fused second half kick): reads position, velocity (32 B),
 #     density, h, mass (12 B); writes acceleration 16 B and velocity 16 B
 ALGO_BYTES = {"density": 28, "force": 76}
+# Algorithmic FP32 operations per neighbour pair (add, mul, div, sqrt and
+# compare count one each; per-particle factors such as p/rho^2, 2 alpha h c
+# and the kernel coefficients counted once per particle, not per pair):
+#   density (Algorithm.cu:17-54), per hit including self: offset 3, d^2 5,
+#     range test 1, sqrt 1, W(d, h_j) 9, rho += m W 2 -> 21
+#   force (Algorithm.cu:56-174), per hit j != i: pressure term 29 (offset 3,
+#     d^2 5, test 1, sqrt 1, direction 3, W'(d, h_j) 7, p term 1, update 8)
+#     + artificial viscosity 23 (velocity difference 3, compression 5, test 1,
+#     nu 2, Pi 4, m Pi W' 2, update 6) -> 52
+FLOPS_PER_PAIR = {"density": 21, "force": 52}
+COUNTERS = ROOT / "profiles" / "counters.json"
 
 
 def log(*a):
@@ -70,24 +90,51 @@ def make_workload(name: str, merged_h: str = "kernel"):
     raise SystemExit(f"unknown workload {name}")
 
 
-def cpu_baseline(scene_obj, steps: int) -> dict:
-    """The oracle (FP32 host restatement, OpenMP) on a bounded sample."""
+def host_facts() -> dict:
+    """CPU model, logical CPUs of the machine and CPUs this process may use."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(scene_obj, steps: int, runs: int) -> dict:
+    """The oracle (FP32 host restatement of the reference's kernels, OpenMP
+    over particles) on a bounded sample of the workload: one warm-up step, then
+    `runs` consecutive runs of `steps` steps each; the median run is reported."""
+    # OpenMP reads OMP_NUM_THREADS when the oracle library loads; default to
+    # every CPU this process may use
+    os.environ.setdefault("OMP_NUM_THREADS", str(len(os.sched_getaffinity(0))))
     from oracle import oracle
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", min(16, os.cpu_count() or 1)))
+    threads = oracle.threads()
     o = oracle.OracleSim.from_scene(scene_obj)
     o.update(scene_obj.dt)  # warm-up step
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        o.update(scene_obj.dt)
-    el = time.perf_counter() - t0
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            o.update(scene_obj.dt)
+        times.append(time.perf_counter() - t0)
+    o.close()
+    med = sorted(times)[len(times) // 2]
+    facts = host_facts()
     return {
-        "value": scene_obj.count * steps / el,
+        "value": scene_obj.count * steps / med,
         "unit": "particle-updates/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{scene_obj.name}: {scene_obj.count} particles, {steps} steps after 1 warm-up step, "
-                  f"{el:.1f} s on {threads} OpenMP threads",
+        "sample": f"{scene_obj.name}: {scene_obj.count} particles, median of {runs} runs of {steps} steps "
+                  f"(after 1 warm-up step; runs {', '.join(f'{t:.1f}' for t in times)} s) on {threads} OpenMP "
+                  f"threads of {facts['nproc']} CPUs ({facts['affinity_cpus']} usable), {facts['cpu_model']}",
+        **facts,
+        "run_s": [round(t, 3) for t in times],
     }
 
 
@@ -101,30 +148,68 @@ def stage_profile(batch, steps: int) -> dict:
     return {k: sum(v) / len(v) for k, v in stages.items()}
 
 
-def roofline_of(stage_ms: dict, particles: int, workload: str) -> dict:
-    # the density stage is the neighbour-list build plus the density kernel
-    stage_ms = dict(stage_ms, density=stage_ms.get("neighbors", 0.0) + stage_ms.get("density", 0.0))
-    dom = max(("density", "force"), key=lambda k: stage_ms.get(k, 0.0))
-    dur = stage_ms[dom] * 1e-3
-    achieved = ALGO_BYTES[dom] * particles / dur / 1e9
-    traffic = None
-    tfile = ROOT / "profiles" / "traffic.json"
-    if tfile.exists():
-        try:
-            t = json.loads(tfile.read_text()).get(workload, {}).get(dom)
-            traffic = t.get("bytes_per_launch") if t else None
-        except (ValueError, AttributeError):
-            traffic = None
+def stored_counters(workload: str) -> dict:
+    """Hardware-counter figures of the workload's neighbour stages
+    (rocprofv3 --pmc runs summarised in profiles/counters.json)."""
+    try:
+        data = json.loads(COUNTERS.read_text())
+    except (OSError, ValueError):
+        return {}
+    out = data.get(workload, {})
+    return {k: dict(v, source=f"profiles/counters.json ({data.get('_build', '?')})") for k, v in out.items()}
+
+
+def stage_figures(stage_ms: dict, particles: int, pairs: float | None, workload: str) -> dict:
+    """Per neighbour stage: time, algorithmic bytes and flops against the HBM
+    and FP32 peaks, and the stored counter figures (measured DRAM traffic,
+    VALU instructions and issue share)."""
+    ms = {"density": stage_ms.get("neighbors", 0.0) + stage_ms.get("density", 0.0),
+          "force": stage_ms.get("force", 0.0)}
+    counters = stored_counters(workload)
+    out = {}
+    for k, t in ms.items():
+        if t <= 0.0:
+            continue
+        sec = t * 1e-3
+        f = {"launch_ms": round(t, 4), "algorithmic_bytes": ALGO_BYTES[k] * particles,
+             "algorithmic_gbs": round(ALGO_BYTES[k] * particles / sec / 1e9, 2)}
+        f["hbm_frac"] = round(f["algorithmic_gbs"] / HBM_PEAK_GBS, 5)
+        if pairs is not None:
+            n_pairs = pairs + (particles if k == "density" else 0)  # the density sums the self term too
+            flops = FLOPS_PER_PAIR[k] * n_pairs
+            f["pairs"] = int(n_pairs)
+            f["flops"] = int(flops)
+            f["tflops"] = round(flops / sec / 1e12, 3)
+            f["fp32_frac"] = round(f["tflops"] / FP32_PEAK_TFLOPS, 4)
+        c = counters.get(k)
+        if c:
+            f["counters"] = c
+        out[k] = f
+    return out
+
+
+def roofline_of(stages: dict) -> dict:
+    """The bench line's roofline object for the dominant kernel: its
+    algorithmic bytes against HBM (the contract's figure), beside the binding
+    resource its counters show (VALU issue) and its algorithmic FP32 rate."""
+    dom = max(stages, key=lambda k: stages[k]["launch_ms"])
+    f = stages[dom]
+    c = f.get("counters", {})
     return {
         "bound": "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 2),
+        "kernel": {"density": "kNeighbors (neighbour lists + density)", "force": "kForce"}[dom],
+        "achieved": f["algorithmic_gbs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5),
-        "traffic": traffic,
-        "algorithmic_bytes_per_launch": ALGO_BYTES[dom] * particles,
-        "launch_ms": round(stage_ms[dom], 4),
+        "frac": f["hbm_frac"],
+        "traffic": c.get("dram_bytes_per_launch"),
+        "algorithmic_bytes_per_launch": f["algorithmic_bytes"],
+        "launch_ms": f["launch_ms"],
+        "binding_resource": "VALU issue (instruction-bound neighbour loop; see valu_issue_frac)",
+        "valu_issue_frac": c.get("valu_issue_frac"),
+        "fp32_tflops": f.get("tflops"),
+        "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+        "fp32_frac": f.get("fp32_frac"),
     }
 
 
@@ -158,6 +243,8 @@ def run_single(args) -> dict:
     batch.synchronize()
     stage_ms = stage_profile(batch, args.profile_steps)
     log("stage ms:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
+    stage_particles = batch.step_particles()
+    pairs = batch.average_neighbor_count(sid) * batch.count(sid)
     batch.synchronize()
     particles = 0
     t0 = time.perf_counter()
@@ -172,25 +259,31 @@ def run_single(args) -> dict:
         particles = s.count * args.steps
     elapsed = time.perf_counter() - t0
     count = batch.count(sid)
+    batch.close()
     value = particles / elapsed
-    metric = "particle-updates/s at 1M WCSPH" if workload == "dam_break_1M" else f"particle-updates/s ({workload})"
+    headline = workload == "dam_break_1M"
+    metric = HEADLINE_METRIC if headline else f"particle-updates/s ({workload})"
     out = base_line(args, metric, value, "particle-updates/s", elapsed / args.steps * 1e3,
                     {"workload": workload, "scenes": 1, "particles": s.count, "final_particles": count,
                      "graph": not args.no_graph, "dt": s.dt,
                      **({"merged_h": args.merged_h} if s.adaptive else {})})
-    out["roofline"] = roofline_of(stage_ms, batch.step_particles(), workload)
-    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(s, args.cpu_steps)
+    stages = stage_figures(stage_ms, stage_particles, pairs, workload)
+    out["roofline"] = roofline_of(stages)
+    out["kernels"] = stages
     out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
-    batch.close()
+    if headline and not args.no_batch:
+        # the second half of the headline metric: 256 scenes stepped together
+        out["batched"] = measure_batch(args, with_sequential=True)
+    out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(s, args.cpu_steps, args.cpu_runs)
     return out
 
 
-def run_batch(args) -> dict:
-    """256 scenes stepped together, and the same scenes one after another
+def measure_batch(args, with_sequential: bool) -> dict:
+    """256 scenes stepped together (graph-captured step, launch shape chosen
+    by the autotuner on a separate copy of the batch), timed over args.steps
+    steps after args.warmup; then, if asked, the same scenes one after another
     (each in its own engine, graph-captured, synchronous steps as in the
-    reference's update()). Both start from the same initial state and time the
-    same steps (warmup .. warmup + steps) of every scene. The launch shape is
-    chosen by the autotuner on a separate copy of the batch."""
+    reference's update()), from the same initial state over the same steps."""
     import numpy as np
 
     from adaptivesph_amd.autotune import autotune
@@ -201,16 +294,16 @@ def run_batch(args) -> dict:
     total0 = sum(s.count for s in scenes)
     log(f"workload {workload}: {len(scenes)} scenes, {total0} particles, "
         f"{sum(s.adaptive for s in scenes)} adaptive ({time.perf_counter() - t0:.1f} s to build)")
-    tuning = Batch.for_scenes(scenes, graph=True, list_capacity=args.list_capacity)
+    lc = args.batch_list_capacity
+    tuning = Batch.for_scenes(scenes, graph=True, list_capacity=lc)
     for s in scenes:
         tuning.add(s)
     tune = autotune(tuning)
     tuning.close()
     log("autotune:", json.dumps(tune))
 
-    batch = Batch.for_scenes(scenes, graph=True, list_capacity=args.list_capacity)
-    for s in scenes:
-        batch.add(s)
+    batch = Batch.for_scenes(scenes, graph=True, list_capacity=lc)
+    sids = [batch.add(s) for s in scenes]
     batch.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
     batch.step(args.warmup)
     batch.synchronize()
@@ -225,43 +318,59 @@ def run_batch(args) -> dict:
     stage_ms = stage_profile(batch, args.profile_steps)
     log("stage ms:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
     stage_particles = batch.step_particles()
+    pairs = sum(batch.average_neighbor_count(i) * batch.count(i) for i in sids)
     batch.close()
-    value = particles / elapsed
-    scene_steps = len(scenes) * args.steps / elapsed
+    out = {
+        "metric": "batched scenes/s + p99 step latency",
+        "value": len(scenes) * args.steps / elapsed,
+        "unit": "scene-steps/s",
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "particle_updates_per_s": particles / elapsed,
+        "step_ms_p50": float(np.percentile(lat, 50)),
+        "step_ms_p99": float(np.percentile(lat, 99)),
+        "latency_steps": len(lat),
+        "config": {"workload": workload, "scenes": len(scenes), "particles": total0,
+                   "adaptive_scenes": sum(s.adaptive for s in scenes), "merged_h": args.merged_h, "graph": True,
+                   "list_capacity": lc,
+                   "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor")}},
+        "autotune": tune["ns_per_particle"],
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "kernels": stage_figures(stage_ms, stage_particles, pairs, workload),
+    }
+    if with_sequential:
+        seq_particles = 0
+        seq_elapsed = 0.0
+        for k, s in enumerate(scenes):
+            b = Batch.for_scenes([s], graph=True, list_capacity=lc)
+            b.add(s)
+            b.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
+            b.step(args.warmup)
+            b.synchronize()
+            t2 = time.perf_counter()
+            for _ in range(args.steps):
+                b.step(1)
+                seq_particles += b.step_particles()
+            seq_elapsed += time.perf_counter() - t2
+            b.close()
+            if k % 64 == 63:
+                log(f"sequential: {k + 1} scenes, {seq_elapsed:.1f} s")
+        out["sequential"] = {"particle_updates_per_s": seq_particles / seq_elapsed,
+                             "scene_steps_per_s": len(scenes) * args.steps / seq_elapsed,
+                             "elapsed_s": seq_elapsed,
+                             "speedup": (particles / elapsed) / (seq_particles / seq_elapsed)}
+    return out
 
-    # the same scenes, one engine per scene, stepped one scene after another
-    seq_particles = 0
-    seq_elapsed = 0.0
-    for k, s in enumerate(scenes):
-        b = Batch.for_scenes([s], graph=True, list_capacity=args.list_capacity)
-        b.add(s)
-        b.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
-        b.step(args.warmup)
-        b.synchronize()
-        t2 = time.perf_counter()
-        for _ in range(args.steps):
-            b.step(1)
-            seq_particles += b.step_particles()
-        seq_elapsed += time.perf_counter() - t2
-        b.close()
-        if k % 64 == 63:
-            log(f"sequential: {k + 1} scenes, {seq_elapsed:.1f} s")
-    seq_value = seq_particles / seq_elapsed
 
-    out = base_line(args, "batched scenes/s + p99 step latency", scene_steps, "scene-steps/s",
-                    elapsed / args.steps * 1e3,
-                    {"workload": workload, "scenes": len(scenes), "particles": total0,
-                     "adaptive_scenes": sum(s.adaptive for s in scenes), "merged_h": args.merged_h, "graph": True,
-                     "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor")}})
-    out["particle_updates_per_s"] = value
-    out["step_ms_p50"] = float(np.percentile(lat, 50))
-    out["step_ms_p99"] = float(np.percentile(lat, 99))
-    out["sequential"] = {"particle_updates_per_s": seq_value, "scene_steps_per_s": len(scenes) * args.steps /
-                         seq_elapsed, "elapsed_s": seq_elapsed, "speedup": value / seq_value}
-    out["autotune"] = tune["ns_per_particle"]
-    out["roofline"] = roofline_of(stage_ms, stage_particles, workload)
+def run_batch(args) -> dict:
+    m = measure_batch(args, with_sequential=True)
+    out = base_line(args, m["metric"], m["value"], m["unit"], m["ms_per_step"], m["config"])
+    for k in ("particle_updates_per_s", "step_ms_p50", "step_ms_p99", "sequential", "autotune", "kernels",
+              "stage_ms"):
+        out[k] = m[k]
+    out["roofline"] = roofline_of(m["kernels"])
     out["cpu_baseline"] = None
-    out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
     return out
 
 
@@ -373,7 +482,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="cfg2")
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=50, help="CPU baseline: steps per run")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs (the median is reported)")
+    ap.add_argument("--no-batch", action="store_true", help="default workload: leave out the batched leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=5)
@@ -387,6 +498,7 @@ def main() -> int:
     ap.add_argument("--merged-h", choices=("kernel", "reference"), default="kernel",
                     help="merged smoothing length of adaptive scenes: kernel-normalised or the reference's")
     args = ap.parse_args()
+    args.batch_list_capacity = args.list_capacity or 640
     if args.list_capacity == 0:
         # the library's defaults: refined regions hold up to ~13x denser
         # daughters, and particles there have several hundred neighbours

@@ -57,6 +57,10 @@ int asph_batch_synchronize(asph_batch* batch);
 int asph_batch_step_timed(asph_batch* batch, float* milliseconds);
 
 int asph_batch_scene_count(asph_batch* batch, uint32_t scene_id, uint32_t* count);
+
+/* calculateAverageNeighborCount (Simulation.cuh:133; SphSimulation.cu:237-247)
+   of one scene of the batch, over the grid of the most recent step. */
+int asph_batch_average_neighbor_count(asph_batch* batch, uint32_t scene_id, float* out);
 int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, asph_vec4* positions,
                           asph_vec4* velocities, asph_vec4* accelerations, float* densities, float* radii,
                           float* smoothing_radii, float* masses);

@@ -67,6 +67,7 @@ def load() -> C.CDLL:
         "oracle_cbrtf": (f32, [f32]),
         "oracle_sincos": (None, [f64, P(f64), P(f64)]),
         "oracle_mix64": (C.c_uint64, [C.c_uint64]),
+        "oracle_threads": (C.c_int, []),
         "oracle_boundary_generate": (u32, [P(abi.Domain), f32, f32, f32, vp]),
         "oracle_split_one": (C.c_int, [vp, u32, u32]),
         "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32, u32]),
@@ -290,4 +291,5 @@ def mix64(z):
 
 
 def threads() -> int:
-    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    """OpenMP threads of the oracle's parallel loops (omp_get_max_threads)."""
+    return int(load().oracle_threads())

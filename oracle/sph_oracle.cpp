@@ -28,6 +28,7 @@
 // Template parameter T = float is the reference arithmetic; T = double is the
 // FP64 oracle used to state the intrinsic FP32 noise floor.
 
+#include <omp.h>
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1906,6 +1907,9 @@ void oracle_cell_ranges(const int32_t* sortedKeys, uint32_t n, uint32_t cells, u
     std::copy(g.cellStart.begin(), g.cellStart.end(), start);
     std::copy(g.cellEnd.begin(), g.cellEnd.end(), end);
 }
+
+// OpenMP threads the oracle's parallel loops use.
+int oracle_threads(void) { return omp_get_max_threads(); }
 
 // Scalar closed forms for the math pins.
 float oracle_wendland_f32(float d, float h) { return oracle::wendland<float>(d, h); }
