@@ -16,7 +16,9 @@ import numpy as np
 from adaptivesph_amd import abi
 
 HERE = Path(__file__).resolve().parent
-LIB = HERE / "build" / "liboracle.so"
+# ASPH_ORACLE_LIB: another build of the oracle (the mutation tests load
+# deliberately broken copies built from sph_oracle.cpp in a temporary folder)
+LIB = Path(os.environ.get("ASPH_ORACLE_LIB", str(HERE / "build" / "liboracle.so")))
 
 _lib = None
 
@@ -31,6 +33,8 @@ def load() -> C.CDLL:
     if _lib is not None:
         return _lib
     if not LIB.exists():
+        if "ASPH_ORACLE_LIB" in os.environ:
+            raise FileNotFoundError(LIB)
         build()
     lib = C.CDLL(str(LIB))
     vp = C.c_void_p
@@ -68,6 +72,7 @@ def load() -> C.CDLL:
         "oracle_sincos": (None, [f64, P(f64), P(f64)]),
         "oracle_mix64": (C.c_uint64, [C.c_uint64]),
         "oracle_threads": (C.c_int, []),
+        "oracle_criteria": (None, [vp, C.c_int, C.c_double, vp]),
         "oracle_boundary_generate": (u32, [P(abi.Domain), f32, f32, f32, vp]),
         "oracle_split_one": (C.c_int, [vp, u32, u32]),
         "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32, u32]),
@@ -217,6 +222,13 @@ class OracleSim:
         info = abi.DensityInfo()
         self._lib.oracle_density_info(self._h, float(threshold), C.byref(info))
         return info
+
+    def criteria(self, phase: str, t: float = 0.0) -> np.ndarray:
+        """Merge (phase "merge") or split ("split") criterion value of every
+        particle of the current state, over a grid of the current positions."""
+        out = np.zeros(max(self.count(), 1), np.float32)
+        self._lib.oracle_criteria(self._h, {"merge": 0, "split": 1}[phase], float(t), out.ctypes.data)
+        return out[: self.count()]
 
     def refinement_trace(self) -> dict:
         t = abi.RefinementTrace()

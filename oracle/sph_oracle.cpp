@@ -19,11 +19,20 @@
 // All three are within 1 ulp of the correctly rounded result (pinned against
 // libm in tests/test_oracle_math.py); CUDA's own ulp pattern is unpinned.
 //
-// Parity pins (see tests/): closed forms of the Wendland kernel and Tait EOS,
-// the interior-lattice density 1.009500783785*rho0 with 81 neighbours,
-// momentum conservation of the pair forces, a two-particle analytic force,
-// KDK free fall, brute-force keys / cell ranges / neighbour sets, split and
-// merge closed forms, stable top-k with ties and sentinels.
+// Parity pins (tests/test_oracle_math.py, test_oracle_physics.py,
+// test_oracle_pins.py): closed forms of the Wendland kernel and Tait EOS, the
+// interior-lattice density 1.009500783785*rho0 with 81 neighbours, momentum
+// conservation, two-particle closed forms of the pressure, artificial
+// viscosity and surface tension terms with h_i != h_j, one fluid particle
+// against one wall particle for the boundary density, pressure and friction,
+// the wall lattice and psi against an FP32 replay and an O(N^2) sum, every
+// criterion at hand-chosen points and threshold edges, the vorticity of a
+// rigid rotation on a lattice, merge pairing (exact tie, non-mutual chain,
+// index-0 reset aliasing), mass-weighted merge, stable compaction, split
+// count rounding and capacity clamp, getDensityInfo on a hand-built array,
+// KDK free fall, brute-force keys / cell ranges / neighbour sets, stable
+// top-k with ties and sentinels. tests/test_oracle_mutants.py checks that 20
+// plausible slips planted in copies of this file each fail one of them.
 //
 // Template parameter T = float is the reference arithmetic; T = double is the
 // FP64 oracle used to state the intrinsic FP32 noise floor.
@@ -1906,6 +1915,22 @@ void oracle_cell_ranges(const int32_t* sortedKeys, uint32_t n, uint32_t cells, u
     }
     std::copy(g.cellStart.begin(), g.cellStart.end(), start);
     std::copy(g.cellEnd.begin(), g.cellEnd.end(), end);
+}
+
+// Criterion values of every particle of the current state (phase 0: merge
+// generators, 1: split generators; getCriterionValues, AdaptiveAlgorithm.cuh:
+// 40-54) over a grid rebuilt from the current positions, with the zone
+// centre at time t.
+void oracle_criteria(oracle_sim* s, int phase, double t, float* out)
+{
+    ORACLE_DISPATCH(s, {
+        S.buildGrid(S.fluid, S.pos, S.n);
+        const auto zc = S.zoneCenterAt(t);
+        for (uint32_t i = 0; i < S.n; ++i)
+        {
+            out[i] = static_cast<float>(phase == 0 ? S.mergeCriterion(i, S.fluid, zc) : S.splitCriterion(i, S.fluid, zc));
+        }
+    });
 }
 
 // OpenMP threads the oracle's parallel loops use.
