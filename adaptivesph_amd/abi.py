@@ -25,6 +25,7 @@ MAX_PARTICLE_CAPACITY = (1 << 28) - 64  # ASPH_MAX_PARTICLE_CAPACITY
 NOALLOC_MAX_WALLS = 1 << 20  # ASPH_NOALLOC_MAX_WALLS
 NOALLOC_MAX_BUCKETS = 1 << 22  # ASPH_NOALLOC_MAX_BUCKETS
 FLAG_KERNEL_MERGED_H = 2
+FLAG_TOLERANCE_MATH = 4
 
 CRITERION = {"velocity": 0, "interface": 1, "vorticity": 2, "zone": 3}
 

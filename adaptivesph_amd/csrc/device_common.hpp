@@ -32,6 +32,7 @@ constexpr uint32_t kSceneFixCellStart = 4U;
 constexpr uint32_t kSceneBoundaryInside = 8U;  // some boundary particle lies in an interior cell
 constexpr uint32_t kSceneKernelMergedH = 16U;  // merged h with the kernel normalisation
 constexpr uint32_t kSceneUniformHM = 32U;      // every particle has h = h0 and m = m0 (plain scenes)
+constexpr uint32_t kSceneToleranceMath = 64U;  // density and force sums in the tolerance mode
 
 // Read-only per-scene description (uploaded when the scene set changes).
 struct SceneDev

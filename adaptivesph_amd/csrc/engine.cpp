@@ -600,6 +600,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
     d.flags = kSceneActive | (adaptive ? kSceneAdaptive : 0U) |
               ((p.flags & ASPH_FLAG_FIX_CELL_START) != 0U ? kSceneFixCellStart : 0U) |
               ((p.flags & ASPH_FLAG_KERNEL_MERGED_H) != 0U ? kSceneKernelMergedH : 0U) |
+              ((p.flags & ASPH_FLAG_TOLERANCE_MATH) != 0U ? kSceneToleranceMath : 0U) |
               // refinement is the only writer of h and m; the uniform loops
               // divide by h0 through its reciprocal, valid for h0 in
               // [2^-40, 2^40]

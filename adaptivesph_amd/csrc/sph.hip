@@ -998,6 +998,109 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
     return rho;
 }
 
+// ---- tolerance mode (kSceneToleranceMath) ----------------------------------
+// The same sums in the same list order, with d from the hardware reciprocal
+// square root (d = d2 * rsq(d2)) and 1 / h_j from the hardware reciprocal:
+// each term within a few ulps of the reference's, the sums within 1e-5 of
+// them. List entries satisfy d2 <= (2 h_j)^2 (q <= 2 up to rounding, where
+// the kernel is 0 to within 1e-29), so the kernels need no range branch; the
+// particle itself (d2 = 0) is evaluated at d2 = kTinyD2, which gives W(0) and
+// a zero gradient term.
+constexpr float kTinyD2 = 1e-30F;
+
+__device__ __forceinline__ float fastDistOverH(float d2, float invh)
+{
+    const float dd = fmaxf(d2, kTinyD2);
+    return (dd * __builtin_amdgcn_rsqf(dd)) * invh;
+}
+
+// (1 - q/2)^4 (2q + 1): the Wendland kernel over its per-particle factor
+__device__ __forceinline__ float fastWendlandShape(float qh)
+{
+    const float t = __builtin_fmaf(-0.5F, qh, 1.0F);
+    const float t2 = t * t;
+    return (t2 * t2) * __builtin_fmaf(2.0F, qh, 1.0F);
+}
+
+template <uint32_t UB>
+__device__ __forceinline__ float listDensityFast(const Ctx& c, const SceneDev& sc, uint32_t lp, uint32_t cnt,
+                                                 const F4& q)
+{
+    float rho = 0.0F;
+    const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
+    if ((sc.flags & kSceneUniformHM) != 0U)
+    {
+        const float invh = 1.0F / sc.h0;
+        const float mc3 = sc.m0 * coefOverH3(sc.h0);
+        uint32_t e = 0;
+        for (; e + UB <= cnt; e += UB)
+        {
+            uint32_t j[UB];
+            float3 o[UB];
+#pragma unroll
+            for (uint32_t u = 0; u < UB; u += 4U)
+            {
+                const uint4 v = loadChunk(listRowAt(row0, e + u));
+                j[u] = v.x;
+                j[u + 1U] = v.y;
+                j[u + 2U] = v.z;
+                j[u + 3U] = v.w;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < UB; ++u)
+            {
+                o[u] = recAt<float3>(c.s0, j[u]);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < UB; ++u)
+            {
+                const float dx = o[u].x - q.x;
+                const float dy = o[u].y - q.y;
+                const float dz = o[u].z - q.z;
+                const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                rho = __builtin_fmaf(mc3, fastWendlandShape(fastDistOverH(d2, invh)), rho);
+            }
+        }
+        for (; e < cnt; ++e)
+        {
+            const float3 o = recAt<float3>(c.s0, listRowAt(row0, e)[e % kChunk]);
+            const float dx = o.x - q.x;
+            const float dy = o.y - q.y;
+            const float dz = o.z - q.z;
+            const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+            rho = __builtin_fmaf(mc3, fastWendlandShape(fastDistOverH(d2, invh)), rho);
+        }
+    }
+    else
+    {
+        for (uint32_t e = 0; e < cnt; e += 4U)
+        {
+            const uint4 v = loadChunk(listRowAt(row0, e));
+            const uint32_t j[4] = {v.x, v.y, v.z, v.w};
+            F4 o[4];
+            F4 hm[4];
+#pragma unroll
+            for (uint32_t u = 0; u < 4U; ++u)
+            {
+                const uint32_t jj = e + u < cnt ? j[u] : j[0];
+                o[u] = recAt<F4>(c.s0, jj);
+                hm[u] = recAt<F4>(c.shm, jj);
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 4U; ++u)
+            {
+                const float dx = o[u].x - q.x;
+                const float dy = o[u].y - q.y;
+                const float dz = o[u].z - q.z;
+                const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+                const float w = hm[u].y * (hm[u].z * fastWendlandShape(fastDistOverH(d2, __builtin_amdgcn_rcpf(hm[u].x))));
+                rho = rho + (e + u < cnt ? w : 0.0F);
+            }
+        }
+    }
+    return rho;
+}
+
 // Neighbour lists and densities (computeDensities, Algorithm.cu:17-54): each
 // thread builds its particle's list, then sums the density over it. BS (the
 // block size) is a compile-time constant so that the staging addresses fold
@@ -1093,7 +1196,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
                 // the density over the list just written (its entries and the
                 // neighbours' records are still in cache), then the boundary
                 // term, the pressure and the force records
-                const float rho = listDensity<kListDensityUniformBatch, kListDensityBatch>(c, sc, pq, cnt, q);
+                const float rho = (sc.flags & kSceneToleranceMath) != 0U
+                                      ? listDensityFast<kListDensityUniformBatch>(c, sc, pq, cnt, q)
+                                      : listDensity<kListDensityUniformBatch, kListDensityBatch>(c, sc, pq, cnt, q);
                 finishDensity(c, sc, b, q, hi, t, rho);
             }
         }
@@ -1495,6 +1600,88 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
     }
 }
 
+// Tolerance mode (kSceneToleranceMath, no surface tension or Morris term):
+// computePressureAccelerations and computeViscosityAccelerations over the
+// list in one sum. With r = x_i - x_j, q = d / h_j and t = 1 - q/2 the
+// kernel gradient along r^ is W'(d, h_j) r / d = (kWDCoef / h_j^4) (t^3 / h_j) r,
+// so both pair terms are multiples of r with the factor
+// m_j (kWDCoef / h_j^5) t^3 (p_i / rho_i^2 + p_j / rho_j^2 + Pi_ij),
+// Pi_ij = -2 alpha h_i c (v_ij . r) / ((rho_i + rho_j)(d^2 + 0.01 h_i^2)) for
+// approaching pairs (v_ij . r < 0) and 0 otherwise. The particle itself
+// (r = 0) adds a zero multiple of r. The sum goes to A.apx..apz.
+template <bool kUniform>
+__device__ __forceinline__ void forceListFast(const Ctx& c, const SceneDev& sc, uint32_t lp, uint32_t cnt,
+                                              const F4& q, const F4& vi, float rhoi, float prri, float nuNum,
+                                              float epsH2, PairAcc& A)
+{
+    const float invh0 = kUniform ? 1.0F / sc.h0 : 0.0F;
+    const float mwd0 = kUniform ? (sc.m0 * dcoefOverH4(sc.h0)) * invh0 : 0.0F;
+    const float negNu = -nuNum;
+    constexpr uint32_t B = kUniform ? kForceUniformBatch : kForceBatch;
+    const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
+    float ax = 0.0F;
+    float ay = 0.0F;
+    float az = 0.0F;
+    for (uint32_t e = 0; e < cnt; e += B)
+    {
+        uint32_t j[B];
+        F4 fa[B];
+        F4 fb[B];
+        F4 hm[B];
+#pragma unroll
+        for (uint32_t u = 0; u < B; u += 4U)
+        {
+            // chunks past the list (the tail of the last batch) are not read:
+            // the list holds listCap entries, a multiple of 4 but not always of B
+            const uint4 v = loadChunk(listRowAt(row0, e + u < cnt ? e + u : e));
+            j[u] = v.x;
+            j[u + 1U] = v.y;
+            j[u + 2U] = v.z;
+            j[u + 3U] = v.w;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u)
+        {
+            // entries past cnt (the tail of the last batch) read the first
+            // entry again; their terms are discarded below
+            const uint32_t jj = e + u < cnt ? j[u] : j[0];
+            fa[u] = recAt<F4>(c.fa, jj);
+            fb[u] = recAt<F4>(c.fb, jj);
+            if (!kUniform)
+            {
+                hm[u] = recAt<F4>(c.shm, jj);
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < B; ++u)
+        {
+            const float ox = q.x - fa[u].x;
+            const float oy = q.y - fa[u].y;
+            const float oz = q.z - fa[u].z;
+            const float d2 = __builtin_fmaf(oz, oz, __builtin_fmaf(oy, oy, ox * ox));
+            const float invh = kUniform ? invh0 : __builtin_amdgcn_rcpf(hm[u].x);
+            const float qh = fastDistOverH(d2, invh);
+            const float t = __builtin_fmaf(-0.5F, qh, 1.0F);
+            const float t3 = (t * t) * t;
+            const float vx = vi.x - fb[u].x;
+            const float vy = vi.y - fb[u].y;
+            const float vz = vi.z - fb[u].z;
+            const float comp = __builtin_fmaf(vz, oz, __builtin_fmaf(vy, oy, vx * ox));
+            const float piv = (negNu * comp) * __builtin_amdgcn_rcpf((rhoi + fb[u].w) * (d2 + epsH2));
+            const float coef = (prri + fa[u].w) + (comp < 0.0F ? piv : 0.0F);
+            const float mwd = kUniform ? mwd0 : (hm[u].y * hm[u].w) * invh;
+            const float sraw = (mwd * t3) * coef;
+            const float sp = e + u < cnt ? sraw : 0.0F;
+            ax = __builtin_fmaf(-sp, ox, ax);
+            ay = __builtin_fmaf(-sp, oy, ay);
+            az = __builtin_fmaf(-sp, oz, az);
+        }
+    }
+    A.apx = ax;
+    A.apy = ay;
+    A.apz = az;
+}
+
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kForce(Ctx c)
 {
     const uint32_t n = *c.nTotal;
@@ -1524,7 +1711,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
         PairAcc A {};
-        if ((sc.flags & kSceneUniformHM) != 0U)
+        if ((sc.flags & kSceneToleranceMath) != 0U && !doST && nuLam == 0.0F)
+        {
+            // pressure and viscosity in one sum (doVisc false: nuNum = 0)
+            if ((sc.flags & kSceneUniformHM) != 0U)
+            {
+                forceListFast<true>(c, sc, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, A);
+            }
+            else
+            {
+                forceListFast<false>(c, sc, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, A);
+            }
+        }
+        else if ((sc.flags & kSceneUniformHM) != 0U)
         {
             // (the flag is set only for h0 in [2^-40, 2^40])
             if (nuLam != 0.0F)

@@ -66,7 +66,17 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_workload(name: str, merged_h: str = "kernel"):
+def make_workload(name: str, merged_h: str = "kernel", math: str = "exact"):
+    scenes, workload = _make_workload(name, merged_h)
+    if math == "tolerance":
+        from adaptivesph_amd import abi
+
+        for s in scenes:
+            s.params.flags |= abi.FLAG_TOLERANCE_MATH
+    return scenes, workload
+
+
+def _make_workload(name: str, merged_h: str):
     from adaptivesph_amd import scene
 
     if name == "cfg2":
@@ -234,7 +244,7 @@ def base_line(args, metric: str, value: float, unit: str, ms_per_step: float, co
 def run_single(args) -> dict:
     from adaptivesph_amd.batch import Batch
 
-    scenes, workload = make_workload(args.workload, args.merged_h)
+    scenes, workload = make_workload(args.workload, args.merged_h, args.math)
     s = scenes[0]
     batch = Batch.for_scenes(scenes, graph=not args.no_graph, list_capacity=args.list_capacity)
     sid = batch.add(s)
@@ -265,7 +275,7 @@ def run_single(args) -> dict:
     metric = HEADLINE_METRIC if headline else f"particle-updates/s ({workload})"
     out = base_line(args, metric, value, "particle-updates/s", elapsed / args.steps * 1e3,
                     {"workload": workload, "scenes": 1, "particles": s.count, "final_particles": count,
-                     "graph": not args.no_graph, "dt": s.dt,
+                     "graph": not args.no_graph, "dt": s.dt, "math": args.math,
                      **({"merged_h": args.merged_h} if s.adaptive else {})})
     stages = stage_figures(stage_ms, stage_particles, pairs, workload)
     out["roofline"] = roofline_of(stages)
@@ -290,7 +300,7 @@ def measure_batch(args, with_sequential: bool) -> dict:
     from adaptivesph_amd.batch import Batch
 
     t0 = time.perf_counter()
-    scenes, workload = make_workload("batch", args.merged_h)
+    scenes, workload = make_workload("batch", args.merged_h, args.math)
     total0 = sum(s.count for s in scenes)
     log(f"workload {workload}: {len(scenes)} scenes, {total0} particles, "
         f"{sum(s.adaptive for s in scenes)} adaptive ({time.perf_counter() - t0:.1f} s to build)")
@@ -333,6 +343,7 @@ def measure_batch(args, with_sequential: bool) -> dict:
         "latency_steps": len(lat),
         "config": {"workload": workload, "scenes": len(scenes), "particles": total0,
                    "adaptive_scenes": sum(s.adaptive for s in scenes), "merged_h": args.merged_h, "graph": True,
+                   "math": args.math,
                    "list_capacity": lc,
                    "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor")}},
         "autotune": tune["ns_per_particle"],
@@ -495,6 +506,9 @@ def main() -> int:
     ap.add_argument("--list-capacity", type=int, default=0,
                     help="neighbour-list entries per particle (0: the library default, 128 for plain "
                          "workloads, 640 with refinement)")
+    ap.add_argument("--math", choices=("exact", "tolerance"), default="exact",
+                    help="density and force sums: bit-exact with the reference's FP32 arithmetic, or the tolerance "
+                         "mode (hardware reciprocal square root and reciprocal; within 1e-5)")
     ap.add_argument("--merged-h", choices=("kernel", "reference"), default="kernel",
                     help="merged smoothing length of adaptive scenes: kernel-normalised or the reference's")
     args = ap.parse_args()

@@ -55,6 +55,18 @@ typedef struct asph_domain {
                                        grows h by (16 pi/21)^(2/3) = 1.79 per
                                        merge; default off = reference */
 
+#define ASPH_FLAG_TOLERANCE_MATH 4u /* density and force sums over the neighbour
+                                      lists with the hardware reciprocal square
+                                      root and reciprocal, pressure and viscosity
+                                      sharing one kernel-gradient factor: within
+                                      1e-5 of the reference's FP32 results
+                                      (forces relative to the sum of the pair
+                                      terms' magnitudes), not bit-exact; cell
+                                      keys, sorted order, neighbour sets and
+                                      refinement decisions are unchanged.
+                                      Scenes with surface tension or the Morris
+                                      term keep the exact sums. Default off */
+
 /* Simulation::Parameters (Simulation.cuh:64-109) + extensions. */
 typedef struct asph_parameters {
     asph_domain domain;

@@ -73,6 +73,7 @@ def load() -> C.CDLL:
         "oracle_mix64": (C.c_uint64, [C.c_uint64]),
         "oracle_threads": (C.c_int, []),
         "oracle_criteria": (None, [vp, C.c_int, C.c_double, vp]),
+        "oracle_force_scale": (None, [vp, vp]),
         "oracle_boundary_generate": (u32, [P(abi.Domain), f32, f32, f32, vp]),
         "oracle_split_one": (C.c_int, [vp, u32, u32]),
         "oracle_merged_h": (f32, [vp, vp, f32, f32, f32, f32, vp, f32, u32]),
@@ -222,6 +223,13 @@ class OracleSim:
         info = abi.DensityInfo()
         self._lib.oracle_density_info(self._h, float(threshold), C.byref(info))
         return info
+
+    def force_scale(self) -> np.ndarray:
+        """S_i = sum over i's fluid neighbours of |pressure term| + |viscosity
+        term| at the current state (the scale of force tolerances)."""
+        out = np.zeros(max(self.count(), 1), np.float64)
+        self._lib.oracle_force_scale(self._h, out.ctypes.data)
+        return out[: self.count()]
 
     def criteria(self, phase: str, t: float = 0.0) -> np.ndarray:
         """Merge (phase "merge") or split ("split") criterion value of every

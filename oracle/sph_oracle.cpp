@@ -798,6 +798,52 @@ public:
         }
     }
 
+    // Absolute-sum scale of each particle's fluid pair forces at the current
+    // state: S_i = sum_j (|pressure term_ij| + |viscosity term_ij|), the
+    // normalisation of force tolerances (interior net forces nearly cancel).
+    void forceScale(double* out) const
+    {
+        const T eps = T(0.01);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t ii = 0; ii < static_cast<int64_t>(n); ++ii)
+        {
+            const auto i = static_cast<uint32_t>(ii);
+            const V4<T> q = pos[i];
+            const V4<T> vi = vel[i];
+            const T rhoi = rho[i];
+            const T hi = h[i];
+            const T pi_ = taitPressure(rhoi, p.rho0, p.c0, p.gamma);
+            double S = 0.0;
+            forEach(fluid, pos, q, T(2) * hi, [&](uint32_t j, const V4<T>& pj) {
+                if (j == i)
+                {
+                    return;
+                }
+                const V4<T> off = sub4(q, pj);
+                const T d2 = dot4(off, off);
+                const T hj = h[j];
+                if (d2 > (T(2) * hj) * (T(2) * hj))
+                {
+                    return;
+                }
+                const T d = std::sqrt(d2);
+                const T rhoj = rho[j];
+                const T dw = wendlandDerivative(d, hj);
+                const T pterm = pi_ / (rhoi * rhoi) + (taitPressure(rhoj, p.rho0, p.c0, p.gamma) / (rhoj * rhoj));
+                S += std::fabs(static_cast<double>(m[j] * dw * pterm));
+                const V4<T> vd = sub4(vi, vel[j]);
+                const T comp = dot4(vd, off);
+                if (p.alpha != T(0) && comp < T(0))
+                {
+                    const T nu = (((T(2) * p.alpha) * hi) * p.c0) / (rhoi + rhoj);
+                    const T piv = ((-nu) * comp) / (d2 + ((eps * hi) * hi));
+                    S += std::fabs(static_cast<double>(m[j] * piv * dw));
+                }
+            });
+            out[i] = S;
+        }
+    }
+
     // ------------------------------------------------------- refinement --
     bool inDomain(const V4<T>& q, T hh, T eps) const
     {
@@ -1932,6 +1978,9 @@ void oracle_criteria(oracle_sim* s, int phase, double t, float* out)
         }
     });
 }
+
+// S_i of every particle (Sim::forceScale) over the grid of the last update.
+void oracle_force_scale(oracle_sim* s, double* out) { ORACLE_DISPATCH(s, S.forceScale(out)); }
 
 // OpenMP threads the oracle's parallel loops use.
 int oracle_threads(void) { return omp_get_max_threads(); }
