@@ -94,6 +94,13 @@ struct F4
     float x, y, z, w;
 };
 
+// Scenes whose fluid pair forces take the tolerance loop (kForceTol): the
+// tolerance mode without surface tension or the Morris term.
+__host__ __device__ inline bool toleranceForce(const SceneDev& sc)
+{
+    return (sc.flags & kSceneToleranceMath) != 0U && sc.sigma == 0.0F && sc.nuLam == 0.0F;
+}
+
 // ------------------------------------------------------------- math --
 
 __device__ __forceinline__ float gmin(float a, float b) { return (b < a) ? b : a; }

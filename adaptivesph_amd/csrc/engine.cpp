@@ -239,8 +239,8 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
     // s0/s1/fa/fb 64 + shm 16 + scene id 4 + neighbour count 4, group boxes 4,
     // neighbour list 4 per entry (capacity rounded up to whole 4-entry
-    // chunks), overflow queue 4, radix histogram and scan tiles < 1.
-    uint64_t perParticle = 64ULL + 16ULL + 88ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 4ULL + 1ULL;
+    // chunks), overflow and wall queues 8, radix histogram and scan tiles < 1.
+    uint64_t perParticle = 64ULL + 16ULL + 88ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 8ULL + 1ULL;
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
@@ -323,6 +323,8 @@ Engine::Engine(const EngineConfig& cfg)
         c.ncount = alloc<uint32_t>(P);
         c.overflowList = alloc<uint32_t>(P);
         c.overflowCount = alloc<uint32_t>(1);
+        c.wallQueue = alloc<uint32_t>(P);
+        c.wallQueueCount = alloc<uint32_t>(1);
         cfg_.listCap = listCapacityRounded(cfg_.listCap);
         c.listCap = cfg_.listCap;
         // wave-blocked rows: whole waves of 64 slots
@@ -385,6 +387,7 @@ Engine::Engine(const EngineConfig& cfg)
         scenes_.resize(S);
         c.nMax = 1;
         c.cExtent = 2;
+        c.forceKinds = kForceExact;
         keyBits_ = 1;
         buildCtx();
     }
@@ -911,23 +914,27 @@ void Engine::updateExtents()
 {
     uint64_t nMax = 0;
     uint64_t cExtent = 2;
+    uint32_t kinds = 0;
     for (const HostScene& hs : scenes_)
     {
         if (hs.active)
         {
             nMax += hs.dev.pCap;
             cExtent = std::max<uint64_t>(cExtent, static_cast<uint64_t>(hs.dev.cellBase) + hs.cellRegion);
+            kinds |= toleranceForce(hs.dev) ? kForceTolerance : kForceExact;
         }
     }
     nMax = std::max<uint64_t>(nMax, 1);
+    kinds = kinds == 0U ? kForceExact : kinds;
     const int bits = bitsFor(cExtent - 1);
-    if (nMax == ctx_.nMax && cExtent == ctx_.cExtent && bits == keyBits_)
+    if (nMax == ctx_.nMax && cExtent == ctx_.cExtent && bits == keyBits_ && kinds == ctx_.forceKinds)
     {
         return;
     }
     // kernels take the context by value, so captured steps are re-captured
     ctx_.nMax = static_cast<uint32_t>(nMax);
     ctx_.cExtent = static_cast<uint32_t>(cExtent);
+    ctx_.forceKinds = kinds;
     keyBits_ = bits;
     buildCtx();
     invalidateGraphs();

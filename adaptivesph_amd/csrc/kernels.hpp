@@ -59,6 +59,9 @@ struct Ctx
     // scenes' capacities), cExtent the cells in use (highest cell slot + 1).
     uint32_t nMax;
     uint32_t cExtent;
+    // force loops present among the active scenes (host-set): kForceExact
+    // (kForce) and kForceTolerance (kForceTol)
+    uint32_t forceKinds;
 
     // neighbour grid
     uint32_t* keysA;
@@ -82,6 +85,10 @@ struct Ctx
     // number (reset by kPrep each step)
     uint32_t* overflowList;
     uint32_t* overflowCount;
+    // particles whose boundary forces are added by kBoundaryForce (sorted
+    // slots; walls within reach), and their number (reset by kPrep each step)
+    uint32_t* wallQueue;
+    uint32_t* wallQueueCount;
     uint32_t* list;  // [listCap * pCap rounded up to 64], 4-entry chunks, wave-blocked: entry k of slot t
                      // at (((t / 64) * (listCap / 4) + k / 4) * 64 + t % 64) * 4 + k % 4
     uint32_t listCap;  // a multiple of 4
@@ -143,6 +150,9 @@ struct LaunchCfg
     uint32_t blockNeighbor = 256;  // neighbour kernels (64, 128 or 256)
     int radixBits = 0;  // 0: chosen from the key width (radixDigitBits)
 };
+
+constexpr uint32_t kForceExact = 1U;
+constexpr uint32_t kForceTolerance = 2U;
 
 // Step stages, in call order.
 void launchPrep(const Ctx& c, hipStream_t s);

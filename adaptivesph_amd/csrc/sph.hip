@@ -182,6 +182,7 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
     {
         *c.nTotal = carry;
         *c.overflowCount = 0U;
+        *c.wallQueueCount = 0U;
         if (c.critCount != nullptr)
         {
             *c.critCount = mcarry;
@@ -594,6 +595,12 @@ __device__ __forceinline__ uint32_t xcdBlock(uint32_t n)
 // (memory-level parallelism against register footprint).
 #ifndef ASPH_FORCE_BATCH
 #define ASPH_FORCE_BATCH 4
+#endif
+#ifndef ASPH_FORCE_TOL_WAVES
+#define ASPH_FORCE_TOL_WAVES 4
+#endif
+#ifndef ASPH_FORCE_TOL_BATCH
+#define ASPH_FORCE_TOL_BATCH 8
 #endif
 #ifndef ASPH_DENSITY_UNIFORM_BATCH
 #define ASPH_DENSITY_UNIFORM_BATCH 8
@@ -1456,11 +1463,9 @@ __device__ __forceinline__ BoundaryAcc boundaryForce(const Ctx& c, const SceneDe
 // friction (BoundaryAlgorithm.cu:53-162), added to g in the reference's launch
 // order (SphSimulation.cu:110-115), then the second half kick (:117) unless the
 // scene refines this step (the kick then follows the refinement).
-// Fluid pair sums of particle t are in A; adds gravity, the fluid terms,
-// the boundary terms, stores acc and rho, then the second half kick.
-__device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, const SceneState& st, const CellBox& b,
-                                            const F4& q, const F4& vi, const F4& ri, float epsH2, bool doVisc,
-                                            bool doST, float nuLam, uint32_t t, const PairAcc& A)
+// g plus the fluid terms of A, in that order.
+__device__ __forceinline__ F4 fluidAcceleration(const SceneDev& sc, const SceneState& st, const F4& vi, bool doVisc,
+                                                bool doST, float nuLam, const PairAcc& A)
 {
     float ax = st.gEff[0];
     float ay = st.gEff[1];
@@ -1487,20 +1492,18 @@ __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, co
         ay = ay + k * A.asy;
         az = az + k * A.asz;
     }
-    BoundaryAcc B {0.F, 0.F, 0.F, 0.F, 0.F, 0.F};
-    if (sc.bCount > 0U && touchesEdge(sc, b) && wallsMayReach(c, t, 2.0F * ri.z))
-    {
-        B = boundaryForce(c, sc, b, q, vi, ri, epsH2);
-    }
-    ax = ax + B.bpx;
-    ay = ay + B.bpy;
-    az = az + B.bpz;
-    ax = ax + B.bfx;
-    ay = ay + B.bfy;
-    az = az + B.bfz;
+    return F4 {ax, ay, az, 0.0F};
+}
+
+// Stores the acceleration and density of sorted slot t, then the second half
+// kick (halfKickVelocities, Algorithm.cu:270-284) unless the scene refines
+// this step.
+__device__ __forceinline__ void storeAndKick(const Ctx& c, const SceneDev& sc, const SceneState& st, const F4& vi,
+                                             float rhoi, uint32_t t, float ax, float ay, float az)
+{
     const uint32_t i = c.sortedVals[t];
     c.acc[i] = F4 {ax, ay, az, 0.0F};
-    c.rho[i] = ri.x;
+    c.rho[i] = rhoi;
     if (st.refineNow == 0U)
     {
         const float hdt = sc.halfDt;
@@ -1518,6 +1521,63 @@ __device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, co
             vw = vw * k;
         }
         c.vel[i] = F4 {vx, vy, vz, vw};
+    }
+}
+
+// The boundary pressure then friction sums added to a (SphSimulation.cu:115),
+// then the store and kick.
+__device__ __forceinline__ void addBoundaryAndFinish(const Ctx& c, const SceneDev& sc, const SceneState& st,
+                                                     const CellBox& b, const F4& q, const F4& vi, const F4& ri,
+                                                     float epsH2, uint32_t t, F4 a)
+{
+    const BoundaryAcc B = boundaryForce(c, sc, b, q, vi, ri, epsH2);
+    float ax = a.x + B.bpx;
+    float ay = a.y + B.bpy;
+    float az = a.z + B.bpz;
+    ax = ax + B.bfx;
+    ay = ay + B.bfy;
+    az = az + B.bfz;
+    storeAndKick(c, sc, st, vi, ri.x, t, ax, ay, az);
+}
+
+__device__ __forceinline__ bool wallsInReach(const Ctx& c, const SceneDev& sc, const CellBox& b, uint32_t t, float hi)
+{
+    return sc.bCount > 0U && touchesEdge(sc, b) && wallsMayReach(c, t, 2.0F * hi);
+}
+
+// Everything after the fluid sums, in one place (the walking kernels).
+__device__ __forceinline__ void finishForce(const Ctx& c, const SceneDev& sc, const SceneState& st, const CellBox& b,
+                                            const F4& q, const F4& vi, const F4& ri, float epsH2, bool doVisc,
+                                            bool doST, float nuLam, uint32_t t, const PairAcc& A)
+{
+    const F4 a = fluidAcceleration(sc, st, vi, doVisc, doST, nuLam, A);
+    if (wallsInReach(c, sc, b, t, ri.z))
+    {
+        addBoundaryAndFinish(c, sc, st, b, q, vi, ri, epsH2, t, a);
+    }
+    else
+    {
+        storeAndKick(c, sc, st, vi, ri.x, t, a.x, a.y, a.z);
+    }
+}
+
+// The list kernels: particles with walls within reach store g plus the fluid
+// terms and join the wall queue (kBoundaryForce adds the boundary terms, in
+// the same order, and kicks); the others finish here. Keeps the boundary walk
+// out of the list loops' register budget.
+__device__ __forceinline__ void finishForceQueued(const Ctx& c, const SceneDev& sc, const SceneState& st,
+                                                  const CellBox& b, const F4& vi, const F4& ri, bool doVisc,
+                                                  bool doST, float nuLam, uint32_t t, const PairAcc& A)
+{
+    const F4 a = fluidAcceleration(sc, st, vi, doVisc, doST, nuLam, A);
+    if (wallsInReach(c, sc, b, t, ri.z))
+    {
+        c.acc[c.sortedVals[t]] = a;
+        c.wallQueue[atomicAdd(c.wallQueueCount, 1U)] = t;
+    }
+    else
+    {
+        storeAndKick(c, sc, st, vi, ri.x, t, a.x, a.y, a.z);
     }
 }
 
@@ -1617,7 +1677,7 @@ __device__ __forceinline__ void forceListFast(const Ctx& c, const SceneDev& sc, 
     const float invh0 = kUniform ? 1.0F / sc.h0 : 0.0F;
     const float mwd0 = kUniform ? (sc.m0 * dcoefOverH4(sc.h0)) * invh0 : 0.0F;
     const float negNu = -nuNum;
-    constexpr uint32_t B = kUniform ? kForceUniformBatch : kForceBatch;
+    constexpr uint32_t B = kUniform ? ASPH_FORCE_TOL_BATCH : kForceBatch;
     const uint4* row0 = reinterpret_cast<const uint4*>(c.list) + listChunk(c, lp, 0U);
     float ax = 0.0F;
     float ay = 0.0F;
@@ -1697,6 +1757,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         }
         const uint32_t s = c.sscene[t];
         const SceneDev& sc = c.scenes[s];
+        if (toleranceForce(sc))
+        {
+            continue;  // kForceTol
+        }
         const SceneState& st = c.states[s];
         const F4 q = c.s0[t];
         const F4 vi = c.s1[t];
@@ -1711,19 +1775,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
         PairAcc A {};
-        if ((sc.flags & kSceneToleranceMath) != 0U && !doST && nuLam == 0.0F)
-        {
-            // pressure and viscosity in one sum (doVisc false: nuNum = 0)
-            if ((sc.flags & kSceneUniformHM) != 0U)
-            {
-                forceListFast<true>(c, sc, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, A);
-            }
-            else
-            {
-                forceListFast<false>(c, sc, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, A);
-            }
-        }
-        else if ((sc.flags & kSceneUniformHM) != 0U)
+        if ((sc.flags & kSceneUniformHM) != 0U)
         {
             // (the flag is set only for h0 in [2^-40, 2^40])
             if (nuLam != 0.0F)
@@ -1746,7 +1798,72 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                 forceList<false>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
         }
-        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
+        finishForceQueued(c, sc, st, b, vi, ri, doVisc, doST, nuLam, t, A);
+    }
+}
+
+// The particles of tolerance-mode scenes (toleranceForce): forceListFast,
+// then the boundary terms and the second half kick as in kForce. A kernel of
+// its own so that its register budget (and occupancy) is not the exact
+// loops'.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASPH_FORCE_TOL_WAVES))) void kForceTol(Ctx c)
+{
+    const uint32_t n = *c.nTotal;
+    const uint32_t K = c.listCap;
+    for (uint32_t pq = xcdBlock(n) * blockDim.x + threadIdx.x; pq < n; pq += gridDim.x * blockDim.x)
+    {
+        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
+        const uint32_t cnt = c.ncount[t];
+        if (cnt > K)
+        {
+            continue;  // kForceOverflow
+        }
+        const uint32_t s = c.sscene[t];
+        const SceneDev& sc = c.scenes[s];
+        if (!toleranceForce(sc))
+        {
+            continue;  // kForce
+        }
+        const SceneState& st = c.states[s];
+        const F4 q = c.s0[t];
+        const F4 vi = c.s1[t];
+        const F4 ri = ownRecord(c, t);
+        const float hi = ri.z;
+        const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
+        const float epsH2 = (0.01F * hi) * hi;
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
+        PairAcc A {};
+        if ((sc.flags & kSceneUniformHM) != 0U)
+        {
+            forceListFast<true>(c, sc, pq, cnt, q, vi, ri.x, ri.y, nuNum, epsH2, A);
+        }
+        else
+        {
+            forceListFast<false>(c, sc, pq, cnt, q, vi, ri.x, ri.y, nuNum, epsH2, A);
+        }
+        // the viscosity is inside A.ap*: no separate term
+        finishForceQueued(c, sc, st, b, vi, ri, false, false, 0.0F, t, A);
+    }
+}
+
+// Boundary pressure and friction of the wall queue's particles, added to the
+// stored g + fluid terms, then the store and kick.
+__global__ __launch_bounds__(256) void kBoundaryForce(Ctx c)
+{
+    const uint32_t m = *c.wallQueueCount;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+    {
+        const uint32_t t = c.wallQueue[k];
+        const uint32_t s = c.sscene[t];
+        const SceneDev& sc = c.scenes[s];
+        const SceneState& st = c.states[s];
+        const F4 q = c.s0[t];
+        const F4 vi = c.s1[t];
+        const F4 ri = ownRecord(c, t);
+        const float hi = ri.z;
+        const float epsH2 = (0.01F * hi) * hi;
+        const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
+        addBoundaryAndFinish(c, sc, st, b, q, vi, ri, epsH2, t, c.acc[c.sortedVals[t]]);
     }
 }
 
@@ -2681,7 +2798,15 @@ void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     // the dispatcher balance them where a capped grid would give each wave a
     // fixed share
     const uint32_t b = lc.blockNeighbor;
-    hipLaunchKernelGGL(kForce, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
+    if ((c.forceKinds & kForceExact) != 0U)
+    {
+        hipLaunchKernelGGL(kForce, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
+    }
+    if ((c.forceKinds & kForceTolerance) != 0U)
+    {
+        hipLaunchKernelGGL(kForceTol, dim3(gridCover(c.nMax, 256U)), dim3(256), 0, s, c);
+    }
+    hipLaunchKernelGGL(kBoundaryForce, dim3(overflowGrid(c)), dim3(256), 0, s, c);
     hipLaunchKernelGGL(kForceOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);
 }
 
