@@ -242,6 +242,16 @@ def _declare(lib: C.CDLL) -> None:
             C.c_int,
             [P(Parameters), vp, u32, vp, u32, P(RefinementParameters), u32, u32, P(vp)],
         ),
+        "asph_create_imported": (
+            C.c_int,
+            [P(Parameters), vp, u32, P(FluidBuffers), P(BoundaryBuffers), vp, u32, P(RefinementParameters), u32, u32,
+             P(vp)],
+        ),
+        "asph_buffer_allocate": (C.c_int, [C.c_size_t, P(DeviceBuffer)]),
+        "asph_buffer_release": (C.c_int, [P(DeviceBuffer)]),
+        "asph_buffer_download": (C.c_int, [P(DeviceBuffer), C.c_size_t, vp, C.c_size_t]),
+        "asph_buffer_upload": (C.c_int, [P(DeviceBuffer), C.c_size_t, vp, C.c_size_t]),
+        "asph_get_boundary_view": (C.c_int, [vp, P(BoundaryView)]),
         "asph_destroy": (C.c_int, [vp]),
         "asph_update": (C.c_int, [vp, f32]),
         "asph_update_domain": (C.c_int, [vp, P(Domain), vp, u32]),
@@ -302,6 +312,32 @@ def _extra_sigs():
         "asph_grid_cells": (C.c_int, [P(Domain), f32, P(u32)]),
         "asph_device_info": (C.c_int, [C.c_char_p, u32]),
     }
+
+
+class DeviceBuffer(C.Structure):
+    """asph_device_buffer: a device pointer and its size in bytes."""
+
+    _fields_ = [("data", C.c_void_p), ("size", C.c_size_t)]
+
+
+class FluidBuffers(C.Structure):
+    """asph_fluid_buffers (FluidParticlesDataImportedBuffer)."""
+
+    _fields_ = [("positions", DeviceBuffer), ("velocities", DeviceBuffer), ("densities", DeviceBuffer),
+                ("radii", DeviceBuffer)]
+
+
+class BoundaryBuffers(C.Structure):
+    """asph_boundary_buffers (BoundaryParticlesDataImportedBuffer)."""
+
+    _fields_ = [("positions", DeviceBuffer), ("radii", DeviceBuffer), ("colors", DeviceBuffer)]
+
+
+class BoundaryView(C.Structure):
+    """asph_boundary_view (BoundaryParticlesData)."""
+
+    _fields_ = [("positions", C.c_void_p), ("colors", C.c_void_p), ("psi_values", C.c_void_p),
+                ("viscosity_coefficients", C.c_void_p), ("radii", C.c_void_p), ("particle_count", C.c_uint32)]
 
 
 class BatchConfig(C.Structure):

@@ -23,6 +23,28 @@ struct asph_simulation
     uint32_t scene = 0;
     bool adaptive = false;
     float lastDt = -1.0F;
+    // BoundaryParticlesData in the boundary domain's order: positions,
+    // colours and radii in the imported boundary buffers or in arrays owned
+    // here; psi and viscosity coefficients owned here
+    asph_vec4* bPos = nullptr;
+    asph_vec4* bColors = nullptr;
+    float* bRadii = nullptr;
+    float* bPsi = nullptr;
+    float* bVisc = nullptr;
+    uint32_t bCap = 0;
+    uint32_t bCount = 0;
+    bool bImported = false;  // positions, colours and radii are the caller's (fixed capacity)
+    float radius = 0.0F;
+    std::vector<void*> owned;
+
+    ~asph_simulation()
+    {
+        engine.reset();  // finishes the simulation's work before its buffers go
+        for (void* q : owned)
+        {
+            (void)hipFree(q);
+        }
+    }
 };
 
 struct asph_batch
@@ -87,6 +109,194 @@ void checkScene(asph::Engine& e, uint32_t id)
     {
         throw asph::Error(ASPH_ERR_INVALID, "no such scene");
     }
+}
+
+void* ownedAlloc(asph_simulation* sim, size_t bytes)
+{
+    void* q = nullptr;
+    asph::hipCheck(hipMalloc(&q, std::max<size_t>(bytes, 1)), "hipMalloc");
+    sim->owned.push_back(q);
+    return q;
+}
+
+// Replaces an owned array by a new one of `bytes` (contents not kept).
+template <class T>
+void ownedRealloc(asph_simulation* sim, T*& p, size_t bytes)
+{
+    T* q = static_cast<T*>(ownedAlloc(sim, bytes));
+    for (auto it = sim->owned.begin(); it != sim->owned.end(); ++it)
+    {
+        if (*it == static_cast<void*>(p))
+        {
+            sim->owned.erase(it);
+            (void)hipFree(p);
+            break;
+        }
+    }
+    p = q;
+}
+
+void needBuffer(const asph_device_buffer& b, size_t bytes, const char* what)
+{
+    if (b.data == nullptr || b.size < bytes)
+    {
+        throw asph::Error(ASPH_ERR_CAPACITY, std::string(what) + " is NULL or smaller than the capacity needs (" +
+                                                 std::to_string(bytes) + " bytes)");
+    }
+}
+
+// Boundary arrays of a new simulation (capacity cap): the imported buffers
+// when given, else arrays owned by the simulation.
+void setupBoundary(asph_simulation* sim, const asph_boundary_buffers* mem, uint32_t cap, float radius)
+{
+    const size_t n = std::max<uint32_t>(cap, 1);
+    sim->bCap = cap;
+    sim->radius = radius;
+    if (mem != nullptr)
+    {
+        needBuffer(mem->positions, sizeof(asph_vec4) * cap, "boundary positions buffer");
+        needBuffer(mem->radii, sizeof(float) * cap, "boundary radii buffer");
+        needBuffer(mem->colors, sizeof(asph_vec4) * cap, "boundary colours buffer");
+        sim->bPos = static_cast<asph_vec4*>(mem->positions.data);
+        sim->bRadii = static_cast<float*>(mem->radii.data);
+        sim->bColors = static_cast<asph_vec4*>(mem->colors.data);
+        sim->bImported = true;
+    }
+    else
+    {
+        sim->bPos = static_cast<asph_vec4*>(ownedAlloc(sim, sizeof(asph_vec4) * n));
+        sim->bRadii = static_cast<float*>(ownedAlloc(sim, sizeof(float) * n));
+        sim->bColors = static_cast<asph_vec4*>(ownedAlloc(sim, sizeof(asph_vec4) * n));
+    }
+    sim->bPsi = static_cast<float*>(ownedAlloc(sim, sizeof(float) * n));
+    sim->bVisc = static_cast<float*>(ownedAlloc(sim, sizeof(float) * n));
+}
+
+// Room for n wall particles: imported boundary buffers have a fixed
+// capacity (ASPH_ERR_CAPACITY beyond it, before anything changes); the
+// simulation's own arrays grow, as its boundary pools do.
+void needBoundaryCapacity(asph_simulation* sim, uint64_t n)
+{
+    if (n <= sim->bCap)
+    {
+        return;
+    }
+    if (sim->bImported || n > UINT32_MAX)
+    {
+        throw asph::Error(ASPH_ERR_CAPACITY, "the domain has " + std::to_string(n) +
+                                                 " boundary particles, beyond the boundary capacity " +
+                                                 std::to_string(sim->bCap) + " of the imported buffers");
+    }
+    ownedRealloc(sim, sim->bPos, sizeof(asph_vec4) * n);
+    ownedRealloc(sim, sim->bColors, sizeof(asph_vec4) * n);
+    ownedRealloc(sim, sim->bRadii, sizeof(float) * n);
+    ownedRealloc(sim, sim->bPsi, sizeof(float) * n);
+    ownedRealloc(sim, sim->bVisc, sizeof(float) * n);
+    sim->bCap = static_cast<uint32_t>(n);
+}
+
+// SphSimulation::initializeBoundaryParticles (SphSimulation.cu:287-333): the
+// radii (the base particle radius), colours (0.3, 0.3, 0.3, 0.2) and
+// viscosity coefficients (domain.friction) of n wall particles; positions and
+// psi from the records when given (else already on the device).
+void writeBoundary(asph_simulation* sim, const asph_boundary_particle* records, uint32_t n, float friction)
+{
+    sim->bCount = n;
+    if (n == 0)
+    {
+        return;
+    }
+    if (records != nullptr)
+    {
+        std::vector<asph_vec4> pos(n);
+        std::vector<float> psi(n);
+        for (uint32_t i = 0; i < n; ++i)
+        {
+            pos[i] = records[i].position;
+            psi[i] = records[i].psi;
+        }
+        asph::hipCheck(hipMemcpy(sim->bPos, pos.data(), sizeof(asph_vec4) * n, hipMemcpyHostToDevice), "upload");
+        asph::hipCheck(hipMemcpy(sim->bPsi, psi.data(), sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+    }
+    const std::vector<float> radii(n, sim->radius);
+    const std::vector<asph_vec4> colors(n, asph_vec4 {0.3F, 0.3F, 0.3F, 0.2F});
+    const std::vector<float> visc(n, friction);
+    asph::hipCheck(hipMemcpy(sim->bRadii, radii.data(), sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+    asph::hipCheck(hipMemcpy(sim->bColors, colors.data(), sizeof(asph_vec4) * n, hipMemcpyHostToDevice), "upload");
+    asph::hipCheck(hipMemcpy(sim->bVisc, visc.data(), sizeof(float) * n, hipMemcpyHostToDevice), "upload");
+}
+
+void createImpl(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
+                const asph_fluid_buffers* fluidMem, const asph_boundary_buffers* boundaryMem,
+                const asph_boundary_particle* boundary, uint32_t boundary_count,
+                const asph_refinement_parameters* refinement, uint32_t max_fluid_capacity,
+                uint32_t max_boundary_capacity, asph_simulation** out)
+{
+    need(params, "params");
+    need(out, "out");
+    *out = nullptr;
+    asph::validateParams(*params);
+    const bool adaptive = refinement != nullptr && refinement->enabled != 0U;
+    const uint32_t cap = adaptive ? refinement->max_particle_count : max_fluid_capacity;
+    if (count > cap)
+    {
+        throw asph::Error(ASPH_ERR_CAPACITY, "particle count exceeds the fluid capacity");
+    }
+    if (cap > asph::kMaxParticleCapacity)
+    {
+        throw asph::Error(ASPH_ERR_CAPACITY, "particle capacity above 2^28 - 64");
+    }
+    if (boundary_count > max_boundary_capacity)
+    {
+        throw asph::Error(ASPH_ERR_CAPACITY, "boundary count exceeds the boundary capacity");
+    }
+    if (boundary_count > 0)
+    {
+        need(boundary, "boundary");
+    }
+    const asph::GridGeom g = asph::makeGrid(params->domain, params->base_smoothing_radius);
+    asph::EngineConfig cfg;
+    cfg.particleCapacity = cap;
+    cfg.boundaryCapacity = max_boundary_capacity;
+    cfg.cellCapacity = static_cast<uint32_t>(g.cells() + 1U);
+    cfg.sceneSlots = 1;
+    cfg.listCap = asph::defaultListCap(adaptive);
+    if (adaptive)
+    {
+        // the list of a very large capacity may not take more than a
+        // quarter of the free device memory (never below the plain
+        // default; particles past the capacity take the walking path)
+        size_t freeBytes = 0;
+        size_t totalBytes = 0;
+        if (hipMemGetInfo(&freeBytes, &totalBytes) == hipSuccess && cap > 0U)
+        {
+            const uint64_t fit = static_cast<uint64_t>(freeBytes) / 4U / (4ULL * cap);
+            const uint64_t k = std::max<uint64_t>(asph::kDefaultListCapPlain, fit & ~3ULL);
+            cfg.listCap = static_cast<uint32_t>(std::min<uint64_t>(cfg.listCap, k));
+        }
+    }
+    if (fluidMem != nullptr)
+    {
+        // FluidParticlesDataImportedBuffer: the simulation's own arrays
+        needBuffer(fluidMem->positions, sizeof(asph_vec4) * cap, "fluid positions buffer");
+        needBuffer(fluidMem->velocities, sizeof(asph_vec4) * cap, "fluid velocities buffer");
+        needBuffer(fluidMem->densities, sizeof(float) * cap, "fluid densities buffer");
+        needBuffer(fluidMem->radii, sizeof(float) * cap, "fluid radii buffer");
+        cfg.importedPos = static_cast<asph::F4*>(fluidMem->positions.data);
+        cfg.importedVel = static_cast<asph::F4*>(fluidMem->velocities.data);
+        cfg.importedRho = static_cast<float*>(fluidMem->densities.data);
+        cfg.importedRad = static_cast<float*>(fluidMem->radii.data);
+    }
+    cfg.adaptive = adaptive;
+    cfg.useGraph = false;
+    auto sim = std::make_unique<asph_simulation>();
+    setupBoundary(sim.get(), boundaryMem, max_boundary_capacity, params->base_particle_radius);
+    sim->engine = std::make_unique<asph::Engine>(cfg);
+    sim->engine->setDebugCapture(adaptive);
+    sim->scene = sim->engine->addScene(*params, positions, count, boundary, boundary_count, refinement, cap, 0.0F);
+    sim->adaptive = adaptive;
+    writeBoundary(sim.get(), boundary, boundary_count, params->domain.friction);
+    *out = sim.release();
 }
 
 }  // namespace
@@ -193,54 +403,90 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
                 uint32_t max_boundary_capacity, asph_simulation** out)
 {
     return guard([&] {
-        need(params, "params");
+        createImpl(params, positions, count, nullptr, nullptr, boundary, boundary_count, refinement,
+                   max_fluid_capacity, max_boundary_capacity, out);
+    });
+}
+
+int asph_create_imported(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
+                         const asph_fluid_buffers* fluid_memory, const asph_boundary_buffers* boundary_memory,
+                         const asph_boundary_particle* boundary, uint32_t boundary_count,
+                         const asph_refinement_parameters* refinement, uint32_t max_fluid_capacity,
+                         uint32_t max_boundary_capacity, asph_simulation** out)
+{
+    return guard([&] {
+        need(fluid_memory, "fluid_memory");
+        need(boundary_memory, "boundary_memory");
+        createImpl(params, positions, count, fluid_memory, boundary_memory, boundary, boundary_count, refinement,
+                   max_fluid_capacity, max_boundary_capacity, out);
+    });
+}
+
+int asph_buffer_allocate(size_t size, asph_device_buffer* out)
+{
+    return guard([&] {
         need(out, "out");
-        *out = nullptr;
-        asph::validateParams(*params);
-        const bool adaptive = refinement != nullptr && refinement->enabled != 0U;
-        const uint32_t cap = adaptive ? refinement->max_particle_count : max_fluid_capacity;
-        if (count > cap)
+        out->data = nullptr;
+        out->size = 0;
+        void* q = nullptr;
+        asph::hipCheck(hipMalloc(&q, std::max<size_t>(size, 1)), "hipMalloc");
+        out->data = q;
+        out->size = size;
+    });
+}
+
+int asph_buffer_release(asph_device_buffer* buffer)
+{
+    return guard([&] {
+        need(buffer, "buffer");
+        if (buffer->data != nullptr)
         {
-            throw asph::Error(ASPH_ERR_CAPACITY, "particle count exceeds the fluid capacity");
+            asph::hipCheck(hipFree(buffer->data), "hipFree");
         }
-        if (cap > asph::kMaxParticleCapacity)
+        buffer->data = nullptr;
+        buffer->size = 0;
+    });
+}
+
+int asph_buffer_download(const asph_device_buffer* buffer, size_t offset, void* host, size_t bytes)
+{
+    return guard([&] {
+        need(buffer, "buffer");
+        need(host, "host");
+        if (offset > buffer->size || bytes > buffer->size - offset)
         {
-            throw asph::Error(ASPH_ERR_CAPACITY, "particle capacity above 2^28 - 64");
+            throw asph::Error(ASPH_ERR_INVALID, "copy beyond the buffer");
         }
-        if (boundary_count > max_boundary_capacity)
+        asph::hipCheck(hipMemcpy(host, static_cast<const char*>(buffer->data) + offset, bytes, hipMemcpyDeviceToHost),
+                       "download");
+    });
+}
+
+int asph_buffer_upload(const asph_device_buffer* buffer, size_t offset, const void* host, size_t bytes)
+{
+    return guard([&] {
+        need(buffer, "buffer");
+        need(host, "host");
+        if (offset > buffer->size || bytes > buffer->size - offset)
         {
-            throw asph::Error(ASPH_ERR_CAPACITY, "boundary count exceeds the boundary capacity");
+            throw asph::Error(ASPH_ERR_INVALID, "copy beyond the buffer");
         }
-        const asph::GridGeom g = asph::makeGrid(params->domain, params->base_smoothing_radius);
-        asph::EngineConfig cfg;
-        cfg.particleCapacity = cap;
-        cfg.boundaryCapacity = max_boundary_capacity;
-        cfg.cellCapacity = static_cast<uint32_t>(g.cells() + 1U);
-        cfg.sceneSlots = 1;
-        cfg.listCap = asph::defaultListCap(adaptive);
-        if (adaptive)
-        {
-            // the list of a very large capacity may not take more than a
-            // quarter of the free device memory (never below the plain
-            // default; particles past the capacity take the walking path)
-            size_t freeBytes = 0;
-            size_t totalBytes = 0;
-            if (hipMemGetInfo(&freeBytes, &totalBytes) == hipSuccess && cap > 0U)
-            {
-                const uint64_t fit = static_cast<uint64_t>(freeBytes) / 4U / (4ULL * cap);
-                const uint64_t k = std::max<uint64_t>(asph::kDefaultListCapPlain, fit & ~3ULL);
-                cfg.listCap = static_cast<uint32_t>(std::min<uint64_t>(cfg.listCap, k));
-            }
-        }
-        cfg.adaptive = adaptive;
-        cfg.useGraph = false;
-        auto sim = std::make_unique<asph_simulation>();
-        sim->engine = std::make_unique<asph::Engine>(cfg);
-        sim->engine->setDebugCapture(adaptive);
-        sim->scene = sim->engine->addScene(*params, positions, count, boundary, boundary_count, refinement, cap,
-                                           0.0F);
-        sim->adaptive = adaptive;
-        *out = sim.release();
+        asph::hipCheck(hipMemcpy(static_cast<char*>(buffer->data) + offset, host, bytes, hipMemcpyHostToDevice),
+                       "upload");
+    });
+}
+
+int asph_get_boundary_view(asph_simulation* sim, asph_boundary_view* out)
+{
+    return guard([&] {
+        need(sim, "sim");
+        need(out, "out");
+        out->positions = sim->bPos;
+        out->colors = sim->bColors;
+        out->psi_values = sim->bPsi;
+        out->viscosity_coefficients = sim->bVisc;
+        out->radii = sim->bRadii;
+        out->particle_count = sim->bCount;
     });
 }
 
@@ -276,7 +522,10 @@ int asph_update_domain(asph_simulation* sim, const asph_domain* domain, const as
         {
             need(boundary, "boundary");
         }
-        engineOf(sim).updateDomain(sim->scene, *domain, boundary, boundary_count);
+        asph::Engine& e = engineOf(sim);
+        needBoundaryCapacity(sim, boundary_count);
+        e.updateDomain(sim->scene, *domain, boundary, boundary_count);
+        writeBoundary(sim, boundary, boundary_count, domain->friction);
     });
 }
 
@@ -284,7 +533,19 @@ int asph_update_domain_walls(asph_simulation* sim, const asph_domain* domain, fl
 {
     return guard([&] {
         need(domain, "domain");
-        engineOf(sim).updateDomainWalls(sim->scene, *domain, particle_spacing);
+        asph::Engine& e = engineOf(sim);
+        // the new wall count, checked against the boundary arrays first
+        asph::WallSequences sq;
+        if (!(particle_spacing > 0.0F) ||
+            asph::wallSequences(*domain, particle_spacing, 1ULL << 31, &sq) != asph::kWallSeqOk)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "particle spacing must be positive and advance the wall lattice");
+        }
+        asph::WallFaces wf;
+        asph::wallFaceTable(sq, 1.0F, &wf);
+        needBoundaryCapacity(sim, wf.count);
+        e.updateDomainWalls(sim->scene, *domain, particle_spacing, reinterpret_cast<asph::F4*>(sim->bPos), sim->bPsi);
+        writeBoundary(sim, nullptr, wf.count, domain->friction);
     });
 }
 

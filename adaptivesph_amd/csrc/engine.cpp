@@ -298,11 +298,11 @@ Engine::Engine(const EngineConfig& cfg)
         nOrderDev_ = alloc<uint32_t>(1);
         c.nOrderPtr = nOrderDev_;
         c.nTotal = alloc<uint32_t>(1);
-        c.pos = alloc<F4>(P);
-        c.vel = alloc<F4>(P);
+        c.pos = cfg_.importedPos != nullptr ? cfg_.importedPos : alloc<F4>(P);
+        c.vel = cfg_.importedVel != nullptr ? cfg_.importedVel : alloc<F4>(P);
         c.acc = alloc<F4>(P);
-        c.rho = alloc<float>(P);
-        c.rad = alloc<float>(P);
+        c.rho = cfg_.importedRho != nullptr ? cfg_.importedRho : alloc<float>(P);
+        c.rad = cfg_.importedRad != nullptr ? cfg_.importedRad : alloc<float>(P);
         c.h = alloc<float>(P);
         c.m = alloc<float>(P);
         c.pCap = P;
@@ -993,7 +993,7 @@ void Engine::updateDomain(uint32_t id, const asph_domain& dom, const asph_bounda
     hipCheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
 }
 
-void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacing)
+void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacing, F4* latticePos, float* latticePsi)
 {
     if (id >= scenes_.size() || !scenes_[id].active)
     {
@@ -1016,8 +1016,8 @@ void Engine::updateDomainWalls(uint32_t id, const asph_domain& dom, float spacin
     {
         SceneDev& d = hs.dev;
         d.bCount = wf.count;
-        const WallTargets out {ctx_.bs0,  ctx_.bvisc, ctx_.bcellOff, ctx_.bbox0,
-                               ctx_.bbox1, ctx_.bgbox0, ctx_.bgbox1, ctx_.bnear2};
+        const WallTargets out {ctx_.bs0,    ctx_.bvisc,  ctx_.bcellOff, ctx_.bbox0,     ctx_.bbox1,
+                               ctx_.bgbox0, ctx_.bgbox1, ctx_.bnear2,   latticePos, latticePsi};
         uint32_t lost = d.bOff;
         bool inside = false;
         wallsOnDevice(wf, d, out, &lost, &inside);

@@ -87,6 +87,13 @@ struct EngineConfig
     bool adaptive = false;          // allocate refinement scratch
     bool useGraph = false;          // capture and replay one step as a hipGraph
     LaunchCfg launch;
+    // Caller-owned canonical arrays (imported particle memory, the buffers the
+    // reference's simulation writes for the renderer), each of particleCapacity
+    // records; null: allocated by the engine. The engine never frees them.
+    F4* importedPos = nullptr;
+    F4* importedVel = nullptr;
+    float* importedRho = nullptr;
+    float* importedRad = nullptr;
 };
 
 // Bytes of device memory a scene of this shape needs inside an engine
@@ -115,7 +122,10 @@ public:
     void updateDomain(uint32_t id, const asph_domain& d, const asph_boundary_particle* boundary, uint32_t nb);
     // updateDomain with the walls of StaticBoundaryDomain::generate(d,
     // spacing, rho0, h0) built on the device (no particle data through the host)
-    void updateDomainWalls(uint32_t id, const asph_domain& d, float spacing);
+    // latticePos / latticePsi (optional, device, at least the new wall count):
+    // the walls' positions and psi in the generator's order.
+    void updateDomainWalls(uint32_t id, const asph_domain& d, float spacing, F4* latticePos = nullptr,
+                           float* latticePsi = nullptr);
     void enableRefinement(uint32_t id);
     void setFrame(uint32_t id, uint32_t frame);
 

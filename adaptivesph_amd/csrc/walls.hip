@@ -573,6 +573,14 @@ void wallsOnDevice(const WallFaces& wf, const SceneDev& d, const WallTargets& ou
     const uint32_t blocks = std::max<uint32_t>(std::min<uint32_t>((n + 255U) / 256U, 4096U), 1U);
     hipLaunchKernelGGL(kWallLattice, dim3(blocks), dim3(256), 0, s, fs, coords.p, n, pos.p);
     wallPsiOnDevice(pos.p, n, wf.buckets, d.h0, d.rho0, psi.p, s);
+    if (out.lattice != nullptr)
+    {
+        hipCheck(hipMemcpyAsync(out.lattice, pos.p, sizeof(F4) * n, hipMemcpyDeviceToDevice, s), "copy walls");
+    }
+    if (out.latticePsi != nullptr)
+    {
+        hipCheck(hipMemcpyAsync(out.latticePsi, psi.p, sizeof(float) * n, hipMemcpyDeviceToDevice, s), "copy psi");
+    }
     hipLaunchKernelGGL(kBoundaryKeys, dim3(blocks), dim3(256), 0, s, pos.p, n, d, keysA.p, valsA.p);
     int bits = 1;
     while (bits < 32 && (1ULL << bits) <= static_cast<uint64_t>(cells))

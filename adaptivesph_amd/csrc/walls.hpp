@@ -106,6 +106,8 @@ struct WallTargets
     F4* bg0;
     F4* bg1;
     float* bnear2;
+    F4* lattice = nullptr;        // optional: positions in the generator's order
+    float* latticePsi = nullptr;  // optional: their psi
 };
 
 // The walls of the domain of scene description d (whose grid, pool offsets,

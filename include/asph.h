@@ -247,6 +247,78 @@ int asph_create(const asph_parameters* params, const asph_vec4* positions, uint3
                 asph_simulation** out);
 int asph_destroy(asph_simulation* sim);
 
+/* Imported particle memory. The reference's caller hands the simulation
+   device buffers it owns (the renderer's Vulkan buffers, imported with
+   importBuffer, ImportedMemory.cuh:10-26), grouped as
+   FluidParticlesDataImportedBuffer and BoundaryParticlesDataImportedBuffer
+   (Simulation.cuh:46-59); the simulation keeps the fluid positions,
+   velocities, densities and radii in those buffers and writes the boundary
+   positions, radii and colours into the boundary ones. Here a buffer is any
+   device allocation of the current GPU: asph_buffer_allocate stands in for
+   importBuffer (there is no Vulkan interop), and a caller's own hipMalloc'd
+   memory serves as well. */
+typedef struct asph_device_buffer {
+    void* data;  /* device pointer */
+    size_t size; /* bytes */
+} asph_device_buffer;
+
+/* FluidParticlesDataImportedBuffer: positions and velocities hold vec4
+   records, densities and radii floats, each at least the fluid capacity. */
+typedef struct asph_fluid_buffers {
+    asph_device_buffer positions;
+    asph_device_buffer velocities;
+    asph_device_buffer densities;
+    asph_device_buffer radii;
+} asph_fluid_buffers;
+
+/* BoundaryParticlesDataImportedBuffer: positions and colours vec4 records,
+   radii floats, each at least the boundary capacity. */
+typedef struct asph_boundary_buffers {
+    asph_device_buffer positions;
+    asph_device_buffer radii;
+    asph_device_buffer colors;
+} asph_boundary_buffers;
+
+/* importBuffer (ImportedMemory.cuh:22-26) replaced by a device allocation of
+   `size` bytes on the current GPU, and its release. */
+int asph_buffer_allocate(size_t size, asph_device_buffer* out);
+int asph_buffer_release(asph_device_buffer* buffer);
+/* Copies between a buffer (from `offset` bytes) and host memory. */
+int asph_buffer_download(const asph_device_buffer* buffer, size_t offset, void* host, size_t bytes);
+int asph_buffer_upload(const asph_device_buffer* buffer, size_t offset, const void* host, size_t bytes);
+
+/* createSimulation with the reference's full signature (Simulation.cuh:
+   138-145): parameters, initial positions, the imported fluid and boundary
+   memory, the static boundary domain's particles, the refinement parameters
+   (NULL = std::nullopt), and the capacities. The fluid buffers become the
+   simulation's own arrays (read and written in place by every update); the
+   boundary buffers receive the wall positions, radii (base_particle_radius)
+   and colours (0.3, 0.3, 0.3, 0.2) in the domain's order, as
+   SphSimulation::initializeBoundaryParticles writes them
+   (SphSimulation.cu:287-333), and again on every updateDomain. The buffers
+   must outlive the simulation. Other arguments as asph_create. */
+int asph_create_imported(const asph_parameters* params, const asph_vec4* positions, uint32_t count,
+                         const asph_fluid_buffers* fluid_memory, const asph_boundary_buffers* boundary_memory,
+                         const asph_boundary_particle* boundary, uint32_t boundary_count,
+                         const asph_refinement_parameters* refinement, uint32_t max_fluid_capacity,
+                         uint32_t max_boundary_capacity, asph_simulation** out);
+
+/* BoundaryParticlesData (Simulation.cuh:35-44): device pointers, in the
+   boundary domain's order (the simulation's neighbour search keeps its own
+   copy sorted by cell). positions, colors and radii are the imported
+   boundary buffers of asph_create_imported, or arrays of the simulation;
+   psi_values and viscosity_coefficients (domain.friction) are the
+   simulation's. Valid until the next updateDomain or asph_destroy. */
+typedef struct asph_boundary_view {
+    asph_vec4* positions;
+    asph_vec4* colors;
+    float* psi_values;
+    float* viscosity_coefficients;
+    float* radii;
+    uint32_t particle_count;
+} asph_boundary_view;
+int asph_get_boundary_view(asph_simulation* sim, asph_boundary_view* out);
+
 /* Simulation::update (Simulation.cuh:126; SphSimulation.cu:101-120,
    AdaptiveSphSimulation.cu:59-85). Synchronous, as in the reference. */
 int asph_update(asph_simulation* sim, float dt);

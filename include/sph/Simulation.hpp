@@ -9,14 +9,20 @@
 // defaults. Differences, all at the edges of that surface:
 //   * glm::vec3 / glm::vec4 are sph::cuda::vec3 / vec4 (same layout) unless
 //     glm is available, in which case they are glm's own types;
-//   * particle memory is allocated and owned by the simulation (there is no
-//     Vulkan buffer import): createSimulation takes no imported buffers, and
-//     fluidParticles() returns the device pointers in FluidParticlesData
-//     layout (the buffers the reference writes into the imported memory);
+//   * ImportedMemory (ImportedMemory.cuh:10-26) is the same abstract base,
+//     but importBuffer(fd, size) (Vulkan external memory) is replaced by
+//     allocateBuffer(size), a device allocation, and wrapBuffer(ptr, size)
+//     for device memory the caller already has; createSimulation takes the
+//     reference's FluidParticlesDataImportedBuffer and
+//     BoundaryParticlesDataImportedBuffer of such buffers (the 8-argument
+//     form), or none (the simulation then owns its memory);
+//   * fluidParticles() / boundaryParticles() return the device pointers in
+//     FluidParticlesData / BoundaryParticlesData layout;
 //   * errors arrive as std::runtime_error carrying the library's message, as
 //     the reference's host failures arrive as exceptions.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <memory>
 #include <optional>
@@ -64,6 +70,108 @@ inline void check(int status, const char* what)
 }
 inline asph_vec3 toC(const vec3& v) { return {v.x, v.y, v.z}; }
 }  // namespace detail
+
+// ImportedMemory (ImportedMemory.cuh:10-26): device memory the caller hands
+// the simulation. devicePointer() and size() describe it; copyToHost and
+// copyFromHost move bytes.
+class ImportedMemory
+{
+public:
+    ImportedMemory() = default;
+    ImportedMemory(const ImportedMemory&) = delete;
+    ImportedMemory(ImportedMemory&&) = delete;
+    ImportedMemory& operator=(const ImportedMemory&) = delete;
+    ImportedMemory& operator=(ImportedMemory&&) = delete;
+    virtual ~ImportedMemory() = default;
+
+    [[nodiscard]] virtual void* devicePointer() const = 0;
+    [[nodiscard]] virtual size_t size() const = 0;
+
+    void copyToHost(void* host, size_t bytes, size_t offset = 0) const
+    {
+        const asph_device_buffer b = buffer();
+        detail::check(asph_buffer_download(&b, offset, host, bytes), "ImportedMemory::copyToHost");
+    }
+    void copyFromHost(const void* host, size_t bytes, size_t offset = 0) const
+    {
+        const asph_device_buffer b = buffer();
+        detail::check(asph_buffer_upload(&b, offset, host, bytes), "ImportedMemory::copyFromHost");
+    }
+    [[nodiscard]] asph_device_buffer buffer() const { return {devicePointer(), size()}; }
+};
+
+namespace detail
+{
+// A device allocation owned by the object (allocateBuffer).
+class DeviceMemory final : public ImportedMemory
+{
+public:
+    explicit DeviceMemory(size_t bytes) { check(asph_buffer_allocate(bytes, &b_), "allocateBuffer"); }
+    ~DeviceMemory() override { (void)asph_buffer_release(&b_); }
+    [[nodiscard]] void* devicePointer() const override { return b_.data; }
+    [[nodiscard]] size_t size() const override { return b_.size; }
+
+private:
+    asph_device_buffer b_ {};
+};
+
+// Device memory owned by someone else (wrapBuffer).
+class BorrowedMemory final : public ImportedMemory
+{
+public:
+    BorrowedMemory(void* p, size_t bytes) : p_(p), n_(bytes) {}
+    [[nodiscard]] void* devicePointer() const override { return p_; }
+    [[nodiscard]] size_t size() const override { return n_; }
+
+private:
+    void* p_;
+    size_t n_;
+};
+}  // namespace detail
+
+// importBuffer (ImportedMemory.cuh:22-26) for a headless program: a device
+// allocation of `size` bytes on the current GPU.
+inline std::unique_ptr<ImportedMemory> allocateBuffer(size_t size)
+{
+    return std::make_unique<detail::DeviceMemory>(size);
+}
+
+// Device memory of the current GPU that the caller owns (it must outlive
+// every simulation it is handed to).
+inline std::unique_ptr<ImportedMemory> wrapBuffer(void* devicePointer, size_t size)
+{
+    return std::make_unique<detail::BorrowedMemory>(devicePointer, size);
+}
+
+// FluidParticlesDataImportedBuffer / BoundaryParticlesDataImportedBuffer
+// (Simulation.cuh:46-59).
+struct FluidParticlesDataImportedBuffer
+{
+    const ImportedMemory& positions;
+    const ImportedMemory& velocities;
+    const ImportedMemory& densities;
+    const ImportedMemory& radii;
+};
+
+struct BoundaryParticlesDataImportedBuffer
+{
+    const ImportedMemory& positions;
+    const ImportedMemory& radii;
+    const ImportedMemory& colors;
+};
+
+// BoundaryParticlesData (Simulation.cuh:35-44): device pointers, in the
+// boundary domain's order.
+struct BoundaryParticlesData
+{
+    vec4* positions;
+    vec4* colors;
+    float* psiValues;
+    float* viscosityCoefficients;
+    float* radii;
+
+    uint32_t particleCount;
+};
 
 // FluidParticlesData (Simulation.cuh:22-33): device pointers.
 struct FluidParticlesData
@@ -357,6 +465,15 @@ public:
                 reinterpret_cast<vec4*>(v.accelerations), v.densities, v.radii, v.smoothing_radii, v.masses,
                 v.particle_count};
     }
+    // The boundary particles (positions, colours and radii are the imported
+    // boundary buffers when the simulation was created with them).
+    [[nodiscard]] BoundaryParticlesData boundaryParticles()
+    {
+        asph_boundary_view v {};
+        detail::check(asph_get_boundary_view(handle_, &v), "boundaryParticles");
+        return {reinterpret_cast<vec4*>(v.positions), reinterpret_cast<vec4*>(v.colors), v.psi_values,
+                v.viscosity_coefficients, v.radii, v.particle_count};
+    }
     // Host download of the positions (and optionally velocities and densities).
     void copyFluid(std::vector<vec4>* positions, std::vector<vec4>* velocities = nullptr,
                    std::vector<float>* densities = nullptr)
@@ -440,9 +557,47 @@ inline void Simulation::updateDomain(const Parameters::Domain& domain,
                   "updateDomain");
 }
 
-// createSimulation (Simulation.cuh:138-145) without the imported renderer
-// buffers; the adaptive variant is chosen when refinementParams->enabled and
-// then uses maxParticleCount as its capacity (AdaptiveSphSimulation.cu:37).
+// createSimulation (Simulation.cuh:138-145, SimulationFactory.cu:14-40): the
+// simulation keeps its fluid positions, velocities, densities and radii in
+// fluidParticleMemory and writes the walls' positions, radii and colours
+// into boundaryParticleMemory (SphSimulation.cu:287-333); the adaptive
+// variant is chosen when refinementParams->enabled and then uses
+// maxParticleCount as its capacity (AdaptiveSphSimulation.cu:37), so the
+// fluid buffers must hold that many particles. The memory must outlive the
+// simulation.
+inline std::unique_ptr<Simulation> createSimulation(
+    const Simulation::Parameters& parameters, const std::vector<vec4>& positions,
+    const FluidParticlesDataImportedBuffer& fluidParticleMemory,
+    const BoundaryParticlesDataImportedBuffer& boundaryParticleMemory,
+    const physics::StaticBoundaryDomain& boundaryDomain,
+    const std::optional<refinement::RefinementParameters>& refinementParams, uint32_t maxFluidParticleCapacity,
+    uint32_t maxBoundaryParticleCapacity)
+{
+    const asph_parameters p = parameters.toC();
+    asph_refinement_parameters r {};
+    const asph_refinement_parameters* rp = nullptr;
+    if (refinementParams.has_value())
+    {
+        r = refinementParams->toC();
+        rp = &r;
+    }
+    const asph_fluid_buffers fluid {fluidParticleMemory.positions.buffer(), fluidParticleMemory.velocities.buffer(),
+                                    fluidParticleMemory.densities.buffer(), fluidParticleMemory.radii.buffer()};
+    const asph_boundary_buffers walls {boundaryParticleMemory.positions.buffer(),
+                                      boundaryParticleMemory.radii.buffer(), boundaryParticleMemory.colors.buffer()};
+    const auto& b = boundaryDomain.getParticles();
+    asph_simulation* h = nullptr;
+    detail::check(asph_create_imported(&p, reinterpret_cast<const asph_vec4*>(positions.data()),
+                                       static_cast<uint32_t>(positions.size()), &fluid, &walls,
+                                       reinterpret_cast<const asph_boundary_particle*>(b.data()),
+                                       static_cast<uint32_t>(b.size()), rp, maxFluidParticleCapacity,
+                                       maxBoundaryParticleCapacity, &h),
+                  "createSimulation");
+    return std::make_unique<Simulation>(h);
+}
+
+// createSimulation without imported memory (the simulation allocates and
+// owns its particle memory); otherwise as above.
 inline std::unique_ptr<Simulation> createSimulation(
     const Simulation::Parameters& parameters, const std::vector<vec4>& positions,
     const physics::StaticBoundaryDomain& boundaryDomain,

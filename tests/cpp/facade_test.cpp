@@ -11,6 +11,12 @@
 //       reads an asph_parameters record and N float4 positions, creates the
 //       simulation through the C++ interface, runs <steps> updates and writes
 //       count, positions, velocities and densities to <out.bin>.
+//   facade_test imported <params.bin> <positions.bin> <steps> <dt> <out.bin> [<refinement.bin>]
+//       the same through the reference's 8-argument createSimulation with
+//       imported particle memory, the call written as App.cpp:154-172 writes
+//       it; the results are read back from the imported buffers (what the
+//       renderer draws): count, positions, velocities, densities, then the
+//       boundary count, positions, radii and colours.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -123,6 +129,190 @@ int host()
     return failures == 0 ? 0 : 1;
 }
 
+// The renderer's particle systems (App.cpp's _api->initializeParticleSystem /
+// initializeBoundaryParticleSystem allocate Vulkan buffers and import them
+// into the simulation): device allocations standing in for them.
+struct ParticleSystem
+{
+    explicit ParticleSystem(uint32_t capacity)
+        : positions(allocateBuffer(capacity * sizeof(vec4))), velocities(allocateBuffer(capacity * sizeof(vec4))),
+          densities(allocateBuffer(capacity * sizeof(float))), radii(allocateBuffer(capacity * sizeof(float)))
+    {
+    }
+    std::unique_ptr<ImportedMemory> positions, velocities, densities, radii;
+};
+
+struct BoundaryParticleSystem
+{
+    explicit BoundaryParticleSystem(uint32_t capacity)
+        : positions(allocateBuffer(capacity * sizeof(vec4))), radii(allocateBuffer(capacity * sizeof(float))),
+          colors(allocateBuffer(capacity * sizeof(vec4)))
+    {
+    }
+    std::unique_ptr<ImportedMemory> positions, radii, colors;
+};
+
+class Api
+{
+public:
+    FluidParticlesDataImportedBuffer initializeParticleSystem(uint32_t capacity)
+    {
+        fluid = std::make_unique<ParticleSystem>(capacity);
+        return {*fluid->positions, *fluid->velocities, *fluid->densities, *fluid->radii};
+    }
+    BoundaryParticlesDataImportedBuffer initializeBoundaryParticleSystem(uint32_t capacity, bool /*render*/)
+    {
+        boundary = std::make_unique<BoundaryParticleSystem>(capacity);
+        return {*boundary->positions, *boundary->radii, *boundary->colors};
+    }
+    std::unique_ptr<ParticleSystem> fluid;
+    std::unique_ptr<BoundaryParticleSystem> boundary;
+};
+
+// the application's initial particle grid (ParticlesParameters::getScalarCount)
+struct InitialParameters
+{
+    uint32_t count;
+    [[nodiscard]] uint32_t getScalarCount() const { return count; }
+};
+
+struct Configuration
+{
+    std::optional<refinement::RefinementParameters> refinementParameters;
+};
+
+refinement::RefinementParameters refinementFromC(const asph_refinement_parameters& c)
+{
+    refinement::RefinementParameters r;
+    r.enabled = c.enabled != 0U;
+    r.minMassRatio = c.min_mass_ratio;
+    r.maxMassRatio = c.max_mass_ratio;
+    r.maxParticleCount = c.max_particle_count;
+    r.maxBatchRatio = c.max_batch_ratio;
+    r.initialCooldown = c.initial_cooldown;
+    r.cooldown = c.cooldown;
+    r.criterionType = static_cast<refinement::RefinementParameters::Criterion>(c.criterion_type);
+    r.splitting.epsilon = c.splitting.epsilon;
+    r.splitting.alpha = c.splitting.alpha;
+    r.splitting.centerMassRatio = c.splitting.center_mass_ratio;
+    r.splitting.vertexMassRatio = c.splitting.vertex_mass_ratio;
+    r.velocity.split.minimalSpeedThreshold = c.velocity.split;
+    r.velocity.merge.maximalSpeedThreshold = c.velocity.merge;
+    r.vorticity.split.minimalVorticityThreshold = c.vorticity.split;
+    r.vorticity.merge.maximalVorticityThreshold = c.vorticity.merge;
+    r.curvature.split.minimalCurvatureThreshold = c.curvature.split;
+    r.curvature.merge.maximalCurvatureThreshold = c.curvature.merge;
+    r.interfaceParameters.split.distanceRatioThreshold = c.interface_ratio.split;
+    r.interfaceParameters.merge.distanceRatioThreshold = c.interface_ratio.merge;
+    r.zone.center = {c.zone.center.x, c.zone.center.y, c.zone.center.z};
+    r.zone.radius = c.zone.radius;
+    r.zone.pathRadius = c.zone.path_radius;
+    r.zone.pathPeriod = c.zone.path_period;
+    return r;
+}
+
+template <class T>
+std::vector<T> download(const ImportedMemory& m, uint32_t n)
+{
+    std::vector<T> v(n);
+    m.copyToHost(v.data(), n * sizeof(T));
+    return v;
+}
+
+int imported(const char* paramsPath, const char* positionsPath, int steps, float dt, const char* outPath,
+             const char* refinementPath)
+{
+    const auto pb = readFile(paramsPath);
+    if (pb.size() != sizeof(asph_parameters))
+    {
+        throw std::runtime_error("parameters file has the wrong size");
+    }
+    asph_parameters c {};
+    std::memcpy(&c, pb.data(), sizeof(c));
+    const auto xb = readFile(positionsPath);
+    std::vector<vec4> _particles(xb.size() / sizeof(vec4));
+    std::memcpy(_particles.data(), xb.data(), _particles.size() * sizeof(vec4));
+    Configuration config;
+    if (refinementPath != nullptr)
+    {
+        const auto rb = readFile(refinementPath);
+        asph_refinement_parameters rc {};
+        if (rb.size() != sizeof(rc))
+        {
+            throw std::runtime_error("refinement file has the wrong size");
+        }
+        std::memcpy(&rc, rb.data(), sizeof(rc));
+        config.refinementParameters = refinementFromC(rc);
+    }
+    const Simulation::Parameters _simulationParameters = fromC(c);
+    const InitialParameters initialParameters {static_cast<uint32_t>(_particles.size())};
+    Api api;
+    Api* _api = &api;
+
+    // App.cpp:143-172
+    const auto particleSpacing = _simulationParameters.baseParticleRadius * 2;
+    const auto boundaryDomain =
+        physics::StaticBoundaryDomain::generate(_simulationParameters.domain,
+                                                particleSpacing,
+                                                _simulationParameters.restDensity,
+                                                _simulationParameters.baseSmoothingRadius);
+
+    const auto maxBoundaryCapacity = boundaryDomain.getParticleCount() * 10;
+
+    // (App.cpp computes the fluid capacity with std::optional::and_then, C++23)
+    const auto refinementCapacity = [](const std::optional<refinement::RefinementParameters>& r) {
+        return r.has_value() && r->enabled ? std::optional<uint32_t>(r->maxParticleCount) : std::nullopt;
+    };
+    auto _simulation = createSimulation(
+        _simulationParameters,
+        _particles,
+        _api->initializeParticleSystem(
+            refinementCapacity(config.refinementParameters).value_or(initialParameters.getScalarCount())),
+        _api->initializeBoundaryParticleSystem(maxBoundaryCapacity, true),
+        boundaryDomain,
+        config.refinementParameters,
+        initialParameters.getScalarCount(),
+        maxBoundaryCapacity);
+
+    for (int k = 0; k < steps; ++k)
+    {
+        _simulation->update(dt);
+    }
+    const uint32_t n = _simulation->getFluidParticlesCount();
+    const uint32_t nb = _simulation->getBoundaryParticlesCount();
+    // the simulation's arrays are the imported buffers
+    const FluidParticlesData view = _simulation->fluidParticles();
+    const BoundaryParticlesData walls = _simulation->boundaryParticles();
+    if (static_cast<void*>(view.positions) != api.fluid->positions->devicePointer() ||
+        static_cast<void*>(view.velocities) != api.fluid->velocities->devicePointer() ||
+        static_cast<void*>(view.densities) != api.fluid->densities->devicePointer() ||
+        static_cast<void*>(view.radii) != api.fluid->radii->devicePointer() ||
+        static_cast<void*>(walls.positions) != api.boundary->positions->devicePointer() ||
+        static_cast<void*>(walls.radii) != api.boundary->radii->devicePointer() ||
+        static_cast<void*>(walls.colors) != api.boundary->colors->devicePointer() || walls.particleCount != nb ||
+        view.particleCount != n)
+    {
+        throw std::runtime_error("the simulation does not use the imported buffers");
+    }
+    const auto x = download<vec4>(*api.fluid->positions, n);
+    const auto v = download<vec4>(*api.fluid->velocities, n);
+    const auto rho = download<float>(*api.fluid->densities, n);
+    const auto bx = download<vec4>(*api.boundary->positions, nb);
+    const auto br = download<float>(*api.boundary->radii, nb);
+    const auto bc = download<vec4>(*api.boundary->colors, nb);
+    std::ofstream f(outPath, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(&n), sizeof(n));
+    f.write(reinterpret_cast<const char*>(x.data()), static_cast<std::streamsize>(n * sizeof(vec4)));
+    f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(n * sizeof(vec4)));
+    f.write(reinterpret_cast<const char*>(rho.data()), static_cast<std::streamsize>(n * sizeof(float)));
+    f.write(reinterpret_cast<const char*>(&nb), sizeof(nb));
+    f.write(reinterpret_cast<const char*>(bx.data()), static_cast<std::streamsize>(nb * sizeof(vec4)));
+    f.write(reinterpret_cast<const char*>(br.data()), static_cast<std::streamsize>(nb * sizeof(float)));
+    f.write(reinterpret_cast<const char*>(bc.data()), static_cast<std::streamsize>(nb * sizeof(vec4)));
+    std::cout << "imported ok: " << n << " particles, " << nb << " boundary particles, " << steps << " steps\n";
+    return 0;
+}
+
 int run(const char* paramsPath, const char* positionsPath, int steps, float dt, const char* outPath)
 {
     const auto pb = readFile(paramsPath);
@@ -183,7 +373,13 @@ int main(int argc, char** argv)
         {
             return run(argv[2], argv[3], std::stoi(argv[4]), std::stof(argv[5]), argv[6]);
         }
-        std::cerr << "usage: facade_test host | run <params.bin> <positions.bin> <steps> <dt> <out.bin>\n";
+        if ((argc == 7 || argc == 8) && std::string(argv[1]) == "imported")
+        {
+            return imported(argv[2], argv[3], std::stoi(argv[4]), std::stof(argv[5]), argv[6],
+                            argc == 8 ? argv[7] : nullptr);
+        }
+        std::cerr << "usage: facade_test host | run|imported <params.bin> <positions.bin> <steps> <dt> <out.bin> "
+                     "[<refinement.bin>]\n";
         return 2;
     }
     catch (const std::exception& e)

@@ -201,3 +201,31 @@ def test_grid_limits_of_the_engine():
     assert lib.asph_grid_cells(C.byref(cube), 0.1, C.byref(n)) == abi.ASPH_ERR_INVALID
     flat = abi.Domain(abi.Vec3(0.0, 0.0, 0.0), abi.Vec3(1.0, 0.0, 1.0), 0.01)
     assert lib.asph_grid_cells(C.byref(flat), 0.1, C.byref(n)) == abi.ASPH_ERR_INVALID
+
+
+def test_imported_buffers_checked_before_device_work():
+    """asph_create_imported refuses imported buffers smaller than the
+    capacities need (or NULL) before it touches the device: the pointers
+    below are never dereferenced."""
+    lib = abi.load_library()
+    s = scene.dam_break((4, 4, 4), 0.01, (0.2, 0.2, 0.2), seed=1, steps=1, boundary="")
+    cap, bcap = s.count, 8
+    fake = 0x1000
+
+    def buf(n):
+        return abi.DeviceBuffer(fake, n)
+
+    good_f = abi.FluidBuffers(buf(16 * cap), buf(16 * cap), buf(4 * cap), buf(4 * cap))
+    good_b = abi.BoundaryBuffers(buf(16 * bcap), buf(4 * bcap), buf(16 * bcap))
+    h = C.c_void_p()
+    for f, b, what in ((abi.FluidBuffers(buf(16 * cap - 1), buf(16 * cap), buf(4 * cap), buf(4 * cap)), good_b,
+                        b"fluid positions"),
+                       (abi.FluidBuffers(buf(16 * cap), buf(16 * cap), buf(4 * cap), abi.DeviceBuffer(None, 4 * cap)),
+                        good_b, b"fluid radii"),
+                       (good_f, abi.BoundaryBuffers(buf(16 * bcap), buf(4 * bcap), buf(16 * bcap - 16)),
+                        b"boundary colours")):
+        rc = lib.asph_create_imported(C.byref(s.params), s.positions.ctypes.data, s.count, C.byref(f), C.byref(b),
+                                      None, 0, None, cap, bcap, C.byref(h))
+        assert rc == abi.ASPH_ERR_CAPACITY and what in lib.asph_last_error(), lib.asph_last_error()
+    assert lib.asph_create_imported(C.byref(s.params), s.positions.ctypes.data, s.count, None, C.byref(good_b), None,
+                                    0, None, cap, bcap, C.byref(h)) == abi.ASPH_ERR_INVALID
