@@ -200,3 +200,22 @@ def test_cfg3_sloshing_tank_at_full_size():
     assert merged > 1000 and split > 100, (merged, split)
     sim.close()
     ref.close()
+
+
+def test_cfg3_reference_merged_h_at_full_size():
+    """The 500,000-particle sloshing tank with the reference's merged
+    smoothing length (flags 0, AdaptiveAlgorithm.cu:50-67): 4 refinement
+    steps, decisions and state bit-exact with the FP32 oracle."""
+    s = scene.cfg3(merged_h="reference")
+    assert s.params.flags == 0
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    for step in range(1, 5):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        _compare_traces(sim.refinement_trace(), ref.refinement_trace(), f"cfg3 reference h step {step}")
+    assert_bitexact(sim.fluid(), ref.fluid(), "cfg3 reference merged h step 4")
+    h = sim.fluid()["smoothing_radii"]
+    assert h.max() > 1.7 * np.float32(s.params.base_smoothing_radius)  # merges grew h by the reference's factor
+    sim.close()
+    ref.close()

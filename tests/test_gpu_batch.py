@@ -141,3 +141,67 @@ def test_benchmark_scene_mix_equals_alone():
     for s, sid in zip(scenes, ids):
         assert_bitexact(b.fluid(sid), _alone(s, 15), f"{s.name}")
     b.close()
+
+
+def _oracle_run(s, steps):
+    from oracle import oracle
+
+    ref = oracle.OracleSim.from_scene(s)
+    for _ in range(steps):
+        ref.update(s.dt)
+    out = ref.fluid()
+    ref.close()
+    return out
+
+
+def test_batch_scenes_equal_oracle():
+    """Six scenes of different kinds in one graph-replayed batch (plain dam
+    break, plain sloshing tank, walls-active dam break, refining dam break
+    with the velocity criterion, refining sloshing tank with the moving zone,
+    refining dam break with the reference's merged h): each equals the FP32
+    oracle stepping that scene alone (SphSimulation::update,
+    AdaptiveSphSimulation::update), bit for bit, after 14 steps."""
+    scenes = _mix()
+    f = scene.dam_break((8, 9, 7), 0.01, (0.2, 0.2, 0.2), seed=47, steps=0, name="f")
+    f.refinement = refinement("velocity", 5000, velocity=(0.02, 0.3), initial_cooldown=1, cooldown=2)
+    f.capacity = 5000
+    scenes.append(f)
+    assert sum(s.adaptive for s in scenes) == 3
+    b = Batch.for_scenes(scenes, graph=True)
+    ids = [b.add(s) for s in scenes]
+    steps = 14
+    b.step(steps)
+    for s, sid in zip(scenes, ids):
+        assert_bitexact(b.fluid(sid), _oracle_run(s, steps), f"{s.name} against the oracle")
+    b.close()
+
+
+def test_cfg4_generator_full_size():
+    """The batched workload's full generator: 256 scenes of 5k-200k particles
+    (about 14M; dam breaks and sloshing tanks, half refining), 3 graph-replayed
+    steps in one batch. Every scene keeps a finite state and a plausible
+    count; 8 sampled scenes equal themselves stepped alone, and two of them
+    (one plain, one refining) equal the FP32 oracle."""
+    scenes = scene.cfg4_scenes(256, steps=3)
+    total = sum(s.count for s in scenes)
+    assert 12_000_000 < total < 16_000_000 and sum(s.adaptive for s in scenes) > 100
+    b = Batch.for_scenes(scenes, graph=True)
+    ids = [b.add(s) for s in scenes]
+    steps = 3
+    b.step(steps)
+    b.synchronize()
+    counts = [b.count(i) for i in ids]
+    for s, n in zip(scenes, counts):
+        assert (n == s.count) if not s.adaptive else (0 < n <= s.capacity), s.name
+    rng = np.random.default_rng(3)
+    sample = sorted(rng.choice(len(scenes), 8, replace=False).tolist())
+    for k in sample:
+        s = scenes[k]
+        got = b.fluid(ids[k])
+        assert all(np.isfinite(got[f]).all() for f in ("positions", "velocities", "densities")), s.name
+        assert_bitexact(got, _alone(s, steps), f"{s.name} alone")
+    plain = next(k for k in sample if not scenes[k].adaptive)
+    refining = next(k for k in sample if scenes[k].adaptive)
+    for k in (plain, refining):
+        assert_bitexact(b.fluid(ids[k]), _oracle_run(scenes[k], steps), f"{scenes[k].name} against the oracle")
+    b.close()

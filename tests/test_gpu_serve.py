@@ -57,3 +57,32 @@ def test_serving_wall_clock_run_completes():
     assert stats["completed"] + stats["rejected"] == len(reqs)
     assert stats["step_ms_p99"] >= stats["step_ms_p50"] > 0
     assert stats["completion_p99"] >= stats["completion_p50"] > 0
+
+
+def test_served_cfg5_sizes_against_alone_and_oracle():
+    """Requests drawn as the serving workload draws them (seeded scenes of
+    log-uniform 10k-500k particles, dam break or sloshing tank, refinement
+    on or off), admitted against the 200 GB pool budget with overlapping
+    arrivals: each result equals the scene stepped alone, and the smallest
+    one equals the FP32 oracle."""
+    from oracle import oracle
+
+    reqs = [serve.Request(rid=i, seed=90_000 + i, steps=st, arrival=arr)
+            for i, (st, arr) in enumerate([(12, 0), (8, 2), (10, 4), (6, 5)])]
+
+    def make(r):
+        return serve.serving_scene(r, merged_h="reference")
+
+    made = [make(r) for r in reqs]
+    assert min(s.count for s in made) >= 10_000 and max(s.count for s in made) <= 500_000
+    srv = serve.Server(200 * 10**9, max_scenes=8, keep_results=True)
+    stats = srv.run(reqs, make, clock=serve.StepClock())
+    srv.close()
+    assert stats["completed"] == len(reqs) and stats["resident_max"] >= 2
+    for r in srv.completed:
+        assert_bitexact(r.result, _alone(make(r), r.steps), f"request {r.rid} ({r.scene.name})", fields=RESULT)
+    small = min(srv.completed, key=lambda r: r.scene.count)
+    ref = oracle.OracleSim.from_scene(make(small))
+    for _ in range(small.steps):
+        ref.update(small.scene.dt)
+    assert_bitexact(small.result, ref.fluid(), f"request {small.rid} against the oracle", fields=RESULT)
