@@ -304,6 +304,7 @@ def _extra_sigs():
         "asph_batch_average_neighbor_count": (C.c_int, [vp, u32, P(f32)]),
         "asph_batch_copy_scene": (C.c_int, [vp, u32, u32, vp, vp, vp, vp, vp, vp, vp]),
         "asph_batch_set_launch": (C.c_int, [vp, u32, u32, u32, u32]),
+        "asph_batch_set_schedule": (C.c_int, [vp, C.c_int32, u32]),
         "asph_batch_memory": (C.c_int, [vp, P(u64), P(u64)]),
         "asph_batch_step_particles": (C.c_int, [vp, P(u32)]),
         "asph_batch_profile_step": (C.c_int, [vp, P(f32), u32, P(u32)]),

@@ -1,12 +1,17 @@
 """Launch-shape autotuner of a batch.
 
-Three launch parameters shape a step, each acting on its own stages:
+Five launch parameters shape a step, each acting on its own stages:
 
 * block_stream   threads per block of the streaming kernels (kick/drift/key,
                  gather);
 * radix_bits     digit width of the cell-key radix sort (cell sort; 0 = the
                  width with the fewest passes);
-* block_neighbor threads per block of the neighbour kernels (density, forces).
+* block_neighbor threads per block of the neighbour kernels (density, forces);
+* query_tile     the cell-tile shape of the neighbour kernels' waves: the
+                 sorted (cell) order, or 2x2x2 / 4x4x4 sub-cell tiles within
+                 each cell (the reordering runs in the gather stage);
+* force_blocks   the force kernels' grid: one thread per particle, or a
+                 persistent grid of 4, 8 or 16 blocks per compute unit.
 
 The tuner measures, in a fixed order, each candidate value on the stages it
 acts on (HIP events between the stages of eager steps of the batch's own
@@ -26,12 +31,16 @@ SPACE = {
     "block_stream": ((128, 256, 512, 1024), ("kick_drift_key", "gather")),
     "radix_bits": ((0, 4, 6, 7, 8, 10), ("cell_sort",)),
     "block_neighbor": ((64, 128, 256), ("neighbors", "density", "force")),
+    "query_tile": ((0, 1, 2), ("gather", "neighbors", "density", "force")),
+    "force_blocks": ((0, 4, 8, 16), ("force",)),
 }
-DEFAULT = {"block_stream": 256, "radix_bits": 0, "block_neighbor": 256}
+# query_tile -1: the engine's own default (sub-cell tiles in adaptive batches)
+DEFAULT = {"block_stream": 256, "radix_bits": 0, "block_neighbor": 256, "query_tile": -1, "force_blocks": 0}
 
 
 def _apply(batch, shape: dict, graph: bool) -> None:
     batch.set_launch(shape["block_stream"], shape["radix_bits"], graph, block_neighbor=shape["block_neighbor"])
+    batch.set_schedule(shape["query_tile"], shape["force_blocks"])
 
 
 def autotune(batch, rounds: int = 3, graph: bool = True, space: dict | None = None) -> dict:

@@ -133,6 +133,13 @@ class Batch:
         self._check(self._lib.asph_batch_set_launch(self._h, block_stream, block_neighbor, radix_bits, int(graph)),
                     "set launch")
 
+    def set_schedule(self, query_tile: int = -1, force_blocks_per_cu: int = 0) -> None:
+        """Query order of the neighbour kernels (0 cell order; 1, 2: 2x2x2 or
+        4x4x4 sub-cell tiles; -1 the default) and the force kernels' grid (0
+        one thread per particle; k: k persistent blocks per compute unit)."""
+        self._check(self._lib.asph_batch_set_schedule(self._h, int(query_tile), int(force_blocks_per_cu)),
+                    "set schedule")
+
     def memory(self) -> tuple[int, int]:
         a = C.c_uint64()
         u = C.c_uint64()

@@ -788,6 +788,25 @@ int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, a
     });
 }
 
+int asph_batch_set_schedule(asph_batch* batch, int32_t query_tile, uint32_t force_blocks_per_cu)
+{
+    return guard([&] {
+        asph::Engine& e = engineOf(batch);
+        if (query_tile < -1 || query_tile > 2)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "query_tile must be -1 (default), 0, 1 or 2");
+        }
+        if (force_blocks_per_cu > 64U)
+        {
+            throw asph::Error(ASPH_ERR_INVALID, "force_blocks_per_cu must be at most 64");
+        }
+        asph::LaunchCfg lc = e.config().launch;
+        lc.queryTile = query_tile;
+        lc.forceBlocksPerCU = force_blocks_per_cu;
+        e.setLaunch(lc);
+    });
+}
+
 int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t block_neighbor, uint32_t radix_bits,
                           uint32_t use_graph)
 {

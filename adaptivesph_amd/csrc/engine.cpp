@@ -239,14 +239,14 @@ uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, b
     // Per fluid slot: canonical arrays 64, sort keys/values 16, sorted shadows
     // s0/s1/fa/fb 64 + shm 16 + scene id 4 + neighbour count 4, group boxes 4,
     // neighbour list 4 per entry (capacity rounded up to whole 4-entry
-    // chunks), overflow and wall queues 8, radix histogram and scan tiles < 1.
-    uint64_t perParticle = 64ULL + 16ULL + 88ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 8ULL + 1ULL;
+    // chunks), overflow and wall queues 8, query order 8, radix histogram and
+    // scan tiles < 1.
+    uint64_t perParticle = 64ULL + 16ULL + 88ULL + 4ULL + 4ULL * listCapacityRounded(listCap) + 8ULL + 8ULL + 1ULL;
     if (adaptive)
     {
         // criterion keys/values (two buffers) 24, candidate/removal/keep/prefix
-        // 16, compaction copies 64, merge list capture 4, sorted slot 4, query
-        // order 8
-        perParticle += 24ULL + 16ULL + 64ULL + 4ULL + 4ULL + 8ULL;
+        // 16, compaction copies 64, merge list capture 4, sorted slot 4
+        perParticle += 24ULL + 16ULL + 64ULL + 4ULL + 4ULL;
     }
     // boundary slot: position + psi 16, viscosity 4, chunk box 32; cell: count,
     // offset and boundary offset 12, boundary box 32, wall distance bound 4.
@@ -337,6 +337,13 @@ Engine::Engine(const EngineConfig& cfg)
         c.bnear2 = alloc<float>(static_cast<size_t>(C) + 1);
         c.bgbox0 = alloc<F4>(B);
         c.bgbox1 = alloc<F4>(B);
+        qpermBuf_ = alloc<uint32_t>(P);
+        qposBuf_ = alloc<uint32_t>(P);
+        int cus = 0;
+        int dev = 0;
+        hipCheck(hipGetDevice(&dev), "hipGetDevice");
+        hipCheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+        computeUnits_ = static_cast<uint32_t>(std::max(cus, 1));
         if (cfg_.adaptive)
         {
             c.ckeysA = alloc<uint64_t>(P);
@@ -346,8 +353,6 @@ Engine::Engine(const EngineConfig& cfg)
             c.cand = alloc<uint32_t>(P);
             c.slotOf = alloc<uint32_t>(P);
             c.critCount = alloc<uint32_t>(1);
-            c.qperm = alloc<uint32_t>(P);
-            c.qpos = alloc<uint32_t>(P);
             c.removal = alloc<uint32_t>(P);
             c.keep = alloc<uint32_t>(P);
             c.rprefix = alloc<uint32_t>(P);
@@ -441,6 +446,11 @@ void Engine::invalidateGraphs()
 void Engine::buildCtx()
 {
     Ctx& c = ctx_;
+    const int tile = cfg_.launch.queryTile < 0 ? (cfg_.adaptive ? 2 : 0) : cfg_.launch.queryTile;
+    c.qperm = tile > 0 ? qpermBuf_ : nullptr;
+    c.qpos = tile > 0 ? qposBuf_ : nullptr;
+    c.queryTileBits = static_cast<uint32_t>(tile);
+    cfg_.launch.forceGrid = cfg_.launch.forceBlocksPerCU * computeUnits_;
     const int rb = radixDigitBits(cfg_.launch.radixBits, keyBits_);
     const int passes = (keyBits_ + rb - 1) / rb;
     sortParity_ = passes & 1;

@@ -217,6 +217,9 @@ private:
     uint32_t* tileSums_ = nullptr;
     uint32_t* scanTotal_ = nullptr;
     uint32_t* nOrderDev_ = nullptr;
+    uint32_t* qpermBuf_ = nullptr;  // query order buffers (Ctx::qperm / qpos when a query tile is set)
+    uint32_t* qposBuf_ = nullptr;
+    uint32_t computeUnits_ = 1;
     uint32_t* cellOrderDev_ = nullptr;
     uint32_t* cellBaseOrderDev_ = nullptr;
     double* infoD_ = nullptr;

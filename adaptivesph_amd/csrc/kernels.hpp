@@ -123,14 +123,15 @@ struct Ctx
     // kPrep; split: by kAfterMerge); the participating scenes' keys are laid
     // out back to back from each scene's segStart
     uint32_t* critCount;
-    // Query order of the list builder and the force kernel (adaptive engines;
-    // null otherwise, meaning the sorted order): query position p handles
-    // sorted slot qperm[p], and slot t's neighbour list sits at query
-    // position qpos[t]. Within each cell the slots are grouped by sub-cell, so
-    // the 64 queries of a wave are near each other even where refinement
-    // packs a cell with daughters in index order.
+    // Query order of the list builder and the force kernels (null: the
+    // sorted order; LaunchCfg::queryTile): query position p handles sorted
+    // slot qperm[p], and slot t's neighbour list sits at query position
+    // qpos[t]. Within each cell the slots are grouped by sub-cell, so the 64
+    // queries of a wave are near each other even where refinement packs a
+    // cell with daughters in index order.
     uint32_t* qperm;
     uint32_t* qpos;
+    uint32_t queryTileBits;  // sub-cell bits per axis of the query order (1 or 2; with qperm)
     uint32_t* removal;
     uint32_t* keep;
     uint32_t* rprefix;
@@ -149,6 +150,15 @@ struct LaunchCfg
     uint32_t blockStream = 256;  // streaming kernels
     uint32_t blockNeighbor = 256;  // neighbour kernels (64, 128 or 256)
     int radixBits = 0;  // 0: chosen from the key width (radixDigitBits)
+    // Query order of the neighbour kernels (kQueryOrder): 0 the sorted (cell)
+    // order; k = 1 or 2: within each cell of more than 64 particles, tiles of
+    // 2^k x 2^k x 2^k sub-cells in Morton order; -1: 2 in adaptive engines,
+    // 0 in plain ones.
+    int queryTile = -1;
+    // Grid of the force kernels: 0 one thread per query position; otherwise a
+    // persistent grid of this many blocks per compute unit (grid-stride loop)
+    uint32_t forceBlocksPerCU = 0;
+    uint32_t forceGrid = 0;  // set by the engine: forceBlocksPerCU x compute units
 };
 
 constexpr uint32_t kForceExact = 1U;

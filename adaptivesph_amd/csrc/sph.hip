@@ -363,12 +363,13 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
     }
 }
 
-// Query order (adaptive engines): within each cell, the slots grouped by
-// their sub-cell (2 bits per axis, Morton order), one wave per cell. Any
-// grouping gives the same lists and sums (each lane's list depends on its
-// own query only); grouping keeps a wave's queries close together, so its
-// candidate filter keeps few survivors even in refined cells, whose
-// particles are in index order (daughters appended per split).
+// Query order (LaunchCfg::queryTile): within each cell of more than 64
+// slots, the slots grouped by their sub-cell (c.queryTileBits bits per axis:
+// 2x2x2 or 4x4x4 sub-cells, in Morton order), one wave per cell. Any grouping
+// gives the same lists and sums (each lane's list depends on its own query
+// only); grouping keeps a wave's queries close together, so its candidate
+// filter keeps few survivors even in refined cells, whose particles are in
+// index order (daughters appended per split).
 __global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
 {
     __shared__ uint32_t bins[4][64];
@@ -394,6 +395,8 @@ __global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
         uint32_t* bin = bins[w];
         bin[lane] = 0U;
         const SceneDev& sc = c.scenes[c.sscene[lo]];
+        const uint32_t bits = c.queryTileBits;  // 1 or 2
+        const uint32_t side = 1U << bits;
         auto subOf = [&](uint32_t t) {
             const F4 q = c.s0[t];
             const float u[3] = {(q.x - sc.dmin[0]) / sc.cs[0], (q.y - sc.dmin[1]) / sc.cs[1],
@@ -403,8 +406,9 @@ __global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
             for (int a = 0; a < 3; ++a)
             {
                 const float f = u[a] - floorf(u[a]);
-                const uint32_t k = min(static_cast<uint32_t>(fmaxf(f, 0.0F) * 4.0F), 3U);
-                m |= ((k & 1U) << a) | ((k >> 1U) << (a + 3));
+                const uint32_t k = min(static_cast<uint32_t>(fmaxf(f, 0.0F) * static_cast<float>(side)), side - 1U);
+                // Morton: bit b of axis a at 3b + a (below 64 bins)
+                m |= ((k & 1U) << a) | (((k >> 1U) & 1U) << (a + 3));
             }
             return m;
         };
@@ -2797,14 +2801,20 @@ void launchForce(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     // refined region's hundreds of neighbours), and many small workgroups let
     // the dispatcher balance them where a capped grid would give each wave a
     // fixed share
+    // (a persistent grid, LaunchCfg::forceBlocksPerCU, is the alternative the
+    // launch-shape tuner weighs against this)
     const uint32_t b = lc.blockNeighbor;
+    auto grid = [&](uint32_t bs) {
+        const uint32_t g = gridCover(c.nMax, bs);
+        return lc.forceGrid != 0U && lc.forceGrid < g ? lc.forceGrid : g;
+    };
     if ((c.forceKinds & kForceExact) != 0U)
     {
-        hipLaunchKernelGGL(kForce, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL(kForce, dim3(grid(b)), dim3(b), 0, s, c);
     }
     if ((c.forceKinds & kForceTolerance) != 0U)
     {
-        hipLaunchKernelGGL(kForceTol, dim3(gridCover(c.nMax, 256U)), dim3(256), 0, s, c);
+        hipLaunchKernelGGL(kForceTol, dim3(grid(256U)), dim3(256), 0, s, c);
     }
     hipLaunchKernelGGL(kBoundaryForce, dim3(overflowGrid(c)), dim3(256), 0, s, c);
     hipLaunchKernelGGL(kForceOverflow, dim3(overflowGrid(c)), dim3(256), 0, s, c);

@@ -315,6 +315,7 @@ def measure_batch(args, with_sequential: bool) -> dict:
     batch = Batch.for_scenes(scenes, graph=True, list_capacity=lc)
     sids = [batch.add(s) for s in scenes]
     batch.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
+    batch.set_schedule(tune["query_tile"], tune["force_blocks"])
     batch.step(args.warmup)
     batch.synchronize()
     particles = 0
@@ -357,6 +358,7 @@ def measure_batch(args, with_sequential: bool) -> dict:
             b = Batch.for_scenes([s], graph=True, list_capacity=lc)
             b.add(s)
             b.set_launch(tune["block_stream"], tune["radix_bits"], True, block_neighbor=tune["block_neighbor"])
+            b.set_schedule(tune["query_tile"], tune["force_blocks"])
             b.step(args.warmup)
             b.synchronize()
             t2 = time.perf_counter()

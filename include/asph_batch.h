@@ -73,6 +73,14 @@ int asph_batch_copy_scene(asph_batch* batch, uint32_t scene_id, uint32_t mask, a
 int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t block_neighbor, uint32_t radix_bits,
                           uint32_t use_graph);
 
+/* Schedule of the neighbour kernels. query_tile orders each wave's particles:
+   0 in sorted (cell) order; 1 or 2 within each cell of more than 64 particles
+   by sub-cell tiles of 2x2x2 or 4x4x4 (Morton order); -1 the default (2 for
+   adaptive batches, 0 otherwise). force_blocks_per_cu: 0 launches the force
+   kernels with one thread per particle, 1..64 as a persistent grid of that
+   many blocks per compute unit. Results do not depend on the schedule. */
+int asph_batch_set_schedule(asph_batch* batch, int32_t query_tile, uint32_t force_blocks_per_cu);
+
 /* Particles of all scenes stepped by the most recent step (their count at
    the start of that step, before refinement). Synchronous. */
 int asph_batch_step_particles(asph_batch* batch, uint32_t* count);

@@ -4,7 +4,7 @@ between steps (continuous batching) do not perturb the others."""
 import numpy as np
 import pytest
 
-from adaptivesph_amd import scene
+from adaptivesph_amd import abi, scene
 from adaptivesph_amd.batch import Batch
 from adaptivesph_amd.simulation import Simulation
 from helpers import assert_bitexact, boundary_scene, refinement
@@ -95,6 +95,41 @@ def test_launch_shapes_do_not_change_results():
             for s, x, y in zip(scenes, out, ref):
                 assert_bitexact(x, y, f"{s.name} block={block} radix={radix} neighbour block={nb}")
         b.close()
+
+
+@pytest.mark.parametrize("tolerance", [False, True])
+def test_schedules_do_not_change_results(tolerance):
+    """Query tiles (cell order, 2x2x2 and 4x4x4 sub-cell tiles) and force
+    grids (one thread per particle, persistent grids) give bit-identical
+    results, for plain and refining scenes (whose cells exceed 64 particles
+    once refined) in both arithmetic modes."""
+    scenes = _mix()
+    for s in scenes:
+        if tolerance:
+            s.params.flags |= abi.FLAG_TOLERANCE_MATH
+    ref = None
+    for tile, blocks in ((-1, 0), (0, 0), (1, 4), (2, 16), (0, 1), (1, 0)):
+        b = Batch.for_scenes(scenes, graph=True)
+        ids = [b.add(s) for s in scenes]
+        b.set_schedule(tile, blocks)
+        b.step(8)
+        out = [b.fluid(i) for i in ids]
+        if ref is None:
+            ref = out
+        else:
+            for s, x, y in zip(scenes, out, ref):
+                assert_bitexact(x, y, f"{s.name} query tile={tile} force blocks per CU={blocks}")
+        b.close()
+
+
+def test_schedule_arguments_are_checked():
+    scenes = _mix()[:1]
+    b = Batch.for_scenes(scenes, graph=False)
+    b.add(scenes[0])
+    for tile, blocks in ((3, 0), (-2, 0), (0, 65)):
+        with pytest.raises(abi.AsphError):
+            b.set_schedule(tile, blocks)
+    b.close()
 
 
 def test_neighbour_grid_covers_more_slots_than_capped_grids():
