@@ -158,6 +158,11 @@ def stage_profile(batch, steps: int) -> dict:
     return {k: sum(v) / len(v) for k, v in stages.items()}
 
 
+def counters_key(workload: str, math: str) -> str:
+    """Key of a workload's entry in profiles/counters.json."""
+    return workload if math == "exact" else f"{workload}/{math}"
+
+
 def stored_counters(workload: str) -> dict:
     """Hardware-counter figures of the workload's neighbour stages
     (rocprofv3 --pmc runs summarised in profiles/counters.json)."""
@@ -207,7 +212,8 @@ def roofline_of(stages: dict) -> dict:
     c = f.get("counters", {})
     return {
         "bound": "hbm",
-        "kernel": {"density": "kNeighbors (neighbour lists + density)", "force": "kForce"}[dom],
+        "kernel": {"density": "kNeighbors (neighbour lists + density)",
+                   "force": "kForce (kForceTol in the tolerance mode)"}[dom],
         "achieved": f["algorithmic_gbs"],
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
@@ -215,7 +221,9 @@ def roofline_of(stages: dict) -> dict:
         "traffic": c.get("dram_bytes_per_launch"),
         "algorithmic_bytes_per_launch": f["algorithmic_bytes"],
         "launch_ms": f["launch_ms"],
-        "binding_resource": "VALU issue (instruction-bound neighbour loop; see valu_issue_frac)",
+        "binding_resource": ("VALU issue (instruction-bound neighbour loop; see valu_issue_frac)"
+                             if (c.get("valu_issue_frac") or 0.0) >= 0.8 else
+                             "vector memory path (per-lane 16-byte gathers through L1; see profiles/valu_*.md)"),
         "valu_issue_frac": c.get("valu_issue_frac"),
         "fp32_tflops": f.get("tflops"),
         "fp32_peak_tflops": FP32_PEAK_TFLOPS,
@@ -277,7 +285,7 @@ def run_single(args) -> dict:
                     {"workload": workload, "scenes": 1, "particles": s.count, "final_particles": count,
                      "graph": not args.no_graph, "dt": s.dt, "math": args.math,
                      **({"merged_h": args.merged_h} if s.adaptive else {})})
-    stages = stage_figures(stage_ms, stage_particles, pairs, workload)
+    stages = stage_figures(stage_ms, stage_particles, pairs, counters_key(workload, args.math))
     out["roofline"] = roofline_of(stages)
     out["kernels"] = stages
     out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
@@ -346,10 +354,11 @@ def measure_batch(args, with_sequential: bool) -> dict:
                    "adaptive_scenes": sum(s.adaptive for s in scenes), "merged_h": args.merged_h, "graph": True,
                    "math": args.math,
                    "list_capacity": lc,
-                   "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor")}},
+                   "launch": {k: tune[k] for k in ("block_stream", "radix_bits", "block_neighbor", "query_tile",
+                                                   "force_blocks")}},
         "autotune": tune["ns_per_particle"],
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
-        "kernels": stage_figures(stage_ms, stage_particles, pairs, workload),
+        "kernels": stage_figures(stage_ms, stage_particles, pairs, counters_key(workload, args.math)),
     }
     if with_sequential:
         seq_particles = 0
@@ -377,11 +386,11 @@ def measure_batch(args, with_sequential: bool) -> dict:
 
 
 def run_batch(args) -> dict:
-    m = measure_batch(args, with_sequential=True)
+    m = measure_batch(args, with_sequential=not args.no_sequential)
     out = base_line(args, m["metric"], m["value"], m["unit"], m["ms_per_step"], m["config"])
     for k in ("particle_updates_per_s", "step_ms_p50", "step_ms_p99", "sequential", "autotune", "kernels",
               "stage_ms"):
-        out[k] = m[k]
+        out[k] = m.get(k)
     out["roofline"] = roofline_of(m["kernels"])
     out["cpu_baseline"] = None
     return out
@@ -500,6 +509,8 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="default workload: leave out the batched leg")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch workload: leave out the per-scene sequential comparison")
     ap.add_argument("--profile-steps", type=int, default=5)
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--rate", type=float, default=2.8, help="serve: requests per second")
