@@ -806,18 +806,63 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
     }
 }
 
-// Density of a particle whose neighbours overflow its list: the walk again,
-// accumulating in the same order. Kept out of line (rare) so that the list
-// loop of kDensity keeps a small register footprint.
-__device__ __forceinline__ float densityWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
-                                          const F4& q)
+// Lane l's value of v (wave-uniform result).
+__device__ __forceinline__ float laneValue(float v, uint32_t l)
 {
-    float rho = 0.0F;
-    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float d2) {
-        const F4 hm = c.shm[j];
-        rho = rho + hm.y * wendlandC(sqrtf(d2), hm.x, hm.z);
-    });
-    return rho;
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), static_cast<int>(l)));
+}
+
+// The walk of one particle by a whole wave (walkFluid's cells, candidates and
+// order; q and b wave-uniform), for particles whose neighbours overflow their
+// list: a cell's slots are taken 64 at a time, one per lane; each lane whose
+// candidate is accepted (d^2 <= (2 h_j)^2) evaluates term(j, d2), then
+// fold(l) runs on every lane for the accepting lanes l in ascending order,
+// which is the visiting order. A sum folded as acc = acc + (lane l's term) is
+// the serial walk's sum bit for bit; the terms, the expensive part, are
+// evaluated 64 at a time.
+template <class T, class F>
+__device__ __forceinline__ void waveWalkFold(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
+                                             const F4& q, uint32_t lane, T&& term, F&& fold)
+{
+    ConstU32* off = (ConstU32*)(c.cellOff);
+    const int gxgy = sc.gx * sc.gy;
+    for (int x = b.x0; x <= b.x1; ++x)
+    {
+        for (int y = b.y0; y <= b.y1; ++y)
+        {
+            for (int z = b.z0; z <= b.z1; ++z)
+            {
+                const uint32_t cell = sc.cellBase + static_cast<uint32_t>(x + y * sc.gx + z * gxgy);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
+                const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+                for (uint32_t j0 = lo; j0 < hi; j0 += 64U)
+                {
+                    const uint32_t j = j0 + lane;
+                    const bool in = j < hi;
+                    const F4 o = c.s0[in ? j : hi - 1U];
+                    const float dx = o.x - q.x;
+                    const float dy = o.y - q.y;
+                    const float dz = o.z - q.z;
+                    const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
+                    const bool acc = in && d2 <= o.w;
+                    uint64_t m = __ballot(acc ? 1 : 0);
+                    if (m == 0ULL)
+                    {
+                        continue;
+                    }
+                    if (acc)
+                    {
+                        term(j, d2);
+                    }
+                    while (m != 0ULL)
+                    {
+                        fold(static_cast<uint32_t>(__builtin_ctzll(m)));
+                        m &= m - 1ULL;
+                    }
+                }
+            }
+        }
+    }
 }
 
 // computeBoundaryDensityContribution (BoundaryAlgorithm.cu:15-51) for one
@@ -1217,11 +1262,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
 }
 
 // Density of the particles whose neighbours overflow their list: the walk
-// again, accumulating in the same order.
+// again (computeDensities, Algorithm.cu:17-54), one wave per particle,
+// accumulating in the same order.
 __global__ __launch_bounds__(256) void kDensityOverflow(Ctx c)
 {
     const uint32_t m = *c.overflowCount;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
+    const uint32_t lane = threadIdx.x & 63U;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64U);
+    for (uint32_t i = blockIdx.x * (blockDim.x / 64U) + threadIdx.x / 64U; i < m; i += waves)
     {
         const uint32_t t = c.overflowList[i];
         const uint32_t s = c.sscene[t];
@@ -1229,8 +1277,19 @@ __global__ __launch_bounds__(256) void kDensityOverflow(Ctx c)
         const F4 q = c.s0[t];
         const float hi = c.shm[t].x;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
-        const float rho = densityWalk(c, sc, c.states[s].lostEnd, b, q);
-        finishDensity(c, sc, b, q, hi, t, rho);
+        float rho = 0.0F;
+        float w = 0.0F;
+        waveWalkFold(
+            c, sc, c.states[s].lostEnd, b, q, lane,
+            [&](uint32_t j, float d2) {
+                const F4 hm = c.shm[j];
+                w = hm.y * wendlandC(sqrtf(d2), hm.x, hm.z);
+            },
+            [&](uint32_t l) { rho = rho + laneValue(w, l); });
+        if (lane == 0U)
+        {
+            finishDensity(c, sc, b, q, hi, t, rho);
+        }
     }
 }
 
@@ -1348,32 +1407,6 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
         A.asy = A.asy + sw * dy;
         A.asz = A.asz + sw * dz;
     }
-}
-
-// Fluid pair forces of a particle whose neighbours overflow its list: the
-// walk again, in the same order. Out of line (rare), like densityWalk.
-__device__ __forceinline__ PairAcc forceWalk(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
-                                          const F4& q, const F4& vi, float rhoi, float prri, float nuNum, float epsH2,
-                                          bool doVisc, bool doST, float nuLam, uint32_t t)
-{
-    PairAcc A {};
-    walkFluid(c, sc, lost, b, q, [&](uint32_t j, const F4&, float) {
-        if (j != t)
-        {
-            const F4 hm = c.shm[j];
-            if (nuLam != 0.0F)
-            {
-                forcePair<false, true>(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
-                                       doST, nuLam, A);
-            }
-            else
-            {
-                forcePair(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam,
-                          A);
-            }
-        }
-    });
-    return A;
 }
 
 struct BoundaryAcc
@@ -1871,10 +1904,14 @@ __global__ __launch_bounds__(256) void kBoundaryForce(Ctx c)
     }
 }
 
+// Forces of the particles whose neighbours overflow their list: the walk
+// again, one wave per particle (waveWalkFold), in the same order.
 __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
 {
     const uint32_t m = *c.overflowCount;
-    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x)
+    const uint32_t lane = threadIdx.x & 63U;
+    const uint32_t waves = gridDim.x * (blockDim.x / 64U);
+    for (uint32_t k = blockIdx.x * (blockDim.x / 64U) + threadIdx.x / 64U; k < m; k += waves)
     {
         const uint32_t t = c.overflowList[k];
         const uint32_t s = c.sscene[t];
@@ -1890,8 +1927,57 @@ __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
         const float nuNum = ((2.0F * sc.alpha) * hi) * sc.c0;
         const float epsH2 = (0.01F * hi) * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, 2.0F * hi);
-        const PairAcc A = forceWalk(c, sc, st.lostEnd, b, q, vi, ri.x, ri.y, nuNum, epsH2, doVisc, doST, nuLam, t);
-        finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
+        // each lane's pair terms: forcePair on zero accumulators gives -t for
+        // the terms it subtracts and t for those it adds, and acc + (-t) is
+        // acc - t exactly (the accumulators start at +0 and never become -0)
+        PairAcc A {};
+        PairAcc Z {};
+        waveWalkFold(
+            c, sc, st.lostEnd, b, q, lane,
+            [&](uint32_t j, float) {
+                Z = PairAcc {};
+                if (j != t)
+                {
+                    const F4 hm = c.shm[j];
+                    if (nuLam != 0.0F)
+                    {
+                        forcePair<false, true>(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, ri.x, ri.y, nuNum, epsH2,
+                                               doVisc, doST, nuLam, Z);
+                    }
+                    else
+                    {
+                        forcePair(c.fa[j], c.fb[j], hm.y, hm.x, hm.w, q, vi, ri.x, ri.y, nuNum, epsH2, doVisc, doST,
+                                  nuLam, Z);
+                    }
+                }
+            },
+            [&](uint32_t l) {
+                A.apx = A.apx + laneValue(Z.apx, l);
+                A.apy = A.apy + laneValue(Z.apy, l);
+                A.apz = A.apz + laneValue(Z.apz, l);
+                if (doVisc)
+                {
+                    A.avx = A.avx + laneValue(Z.avx, l);
+                    A.avy = A.avy + laneValue(Z.avy, l);
+                    A.avz = A.avz + laneValue(Z.avz, l);
+                }
+                if (doST)
+                {
+                    A.asx = A.asx + laneValue(Z.asx, l);
+                    A.asy = A.asy + laneValue(Z.asy, l);
+                    A.asz = A.asz + laneValue(Z.asz, l);
+                }
+                if (nuLam != 0.0F)
+                {
+                    A.amx = A.amx + laneValue(Z.amx, l);
+                    A.amy = A.amy + laneValue(Z.amy, l);
+                    A.amz = A.amz + laneValue(Z.amz, l);
+                }
+            });
+        if (lane == 0U)
+        {
+            finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
+        }
     }
 }
 
@@ -2210,25 +2296,21 @@ __global__ void kCopyMergeList(Ctx c)
 __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
 {
     const uint32_t n = *c.critCount;  // the sorted merge keys
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    const uint32_t lane = threadIdx.x & 63U;
+    // waves of 64 consecutive keys (so that a wave can walk for one lane)
+    for (uint32_t t0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63U); t0 < n; t0 += gridDim.x * blockDim.x)
     {
-        const uint64_t key = c.csortedKeys[t];
-        if (key == ~0ULL)
-        {
-            continue;
-        }
-        const uint32_t s = static_cast<uint32_t>(key >> 32U);
+        const uint32_t t = t0 + lane;
+        const uint64_t key = t < n ? c.csortedKeys[t] : ~0ULL;
+        const uint32_t s = key == ~0ULL ? 0U : static_cast<uint32_t>(key >> 32U);
         const SceneDev& sc = c.scenes[s];
         const SceneState& st = c.states[s];
         const uint32_t e = t - st.segStart;
-        if (e >= st.mergeK)
-        {
-            continue;
-        }
-        const uint32_t id = c.csortedVals[t];
+        const bool active = key != ~0ULL && e < st.mergeK;
+        const uint32_t id = active ? c.csortedVals[t] : 0U;
         const uint32_t i = sc.pOff + id;
-        const F4 q = c.pos[i];
-        const float hh = c.h[i];
+        const F4 q = active ? c.pos[i] : F4 {};
+        const float hh = active ? c.h[i] : 0.0F;
         const float hh2 = hh * hh;
         const float massLimit = sc.maxMassRatio * sc.m0;
         float closest = FLT_MAX;
@@ -2238,7 +2320,7 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
         // them between the gather and the merge), in the reference's order.
         // d2 < hmin^2 implies d2 < h_i^2, so the shadow's h and m are read only
         // past that test, and the candidate's identity only on acceptance.
-        const uint32_t ti = c.slotOf[i];
+        const uint32_t ti = active ? c.slotOf[i] : 0U;
         // d2 of slot when it is a closer admissible candidate, else -1
         auto visit = [&](uint32_t slot) {
             const F4 o = c.s0[slot];
@@ -2257,8 +2339,8 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
             }
             return -1.0F;
         };
-        const uint32_t cnt = c.ncount[ti];
-        if (cnt <= c.listCap)
+        const uint32_t cnt = active ? c.ncount[ti] : 0U;
+        if (active && cnt <= c.listCap)
         {
             const uint32_t lpi = c.qperm != nullptr ? c.qpos[ti] : ti;  // the list's query position
             // Every hit of the walk over the cells within h_i is in the
@@ -2267,9 +2349,9 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
             // the walk order (cells x, y, z, ascending slot) of a larger cell
             // box: the list, restricted to the cells within h_i, is the walk.
             const CellBox b = cellBox(sc, q.x, q.y, q.z, hh);
-            for (uint32_t e = 0; e < cnt; ++e)
+            for (uint32_t k = 0; k < cnt; ++k)
             {
-                const uint32_t slot = *listAt(c, lpi, e);
+                const uint32_t slot = *listAt(c, lpi, k);
                 const float d2 = visit(slot);
                 if (d2 >= 0.0F)
                 {
@@ -2285,18 +2367,79 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
                 }
             }
         }
-        else
+        // Lists that overflowed: the walk over the cells within h_i
+        // (walkSlots' cells and order), by the whole wave for one lane at a
+        // time, 64 slots per step; the admissible slots of a step are then
+        // taken in ascending order, a closer one replacing the best.
+        uint64_t walkers = __ballot((active && cnt > c.listCap) ? 1 : 0);
+        while (walkers != 0ULL)
         {
-            walkSlots(c, sc, st.lostEnd, q.x, q.y, q.z, hh, [&](uint32_t slot) {
-                const float d2 = visit(slot);
-                if (d2 >= 0.0F)
+            const int l = static_cast<int>(__builtin_ctzll(walkers));
+            walkers &= walkers - 1ULL;
+            const uint32_t sl = __builtin_amdgcn_readlane(s, l);
+            const SceneDev& scl = c.scenes[sl];
+            const F4 ql {laneValue(q.x, l), laneValue(q.y, l), laneValue(q.z, l), 0.0F};
+            const float hl = laneValue(hh, l);
+            const float hl2 = hl * hl;
+            const uint32_t tl = __builtin_amdgcn_readlane(ti, l);
+            const float limitl = scl.maxMassRatio * scl.m0;
+            const uint32_t lost = c.states[sl].lostEnd;
+            const CellBox b = cellBox(scl, ql.x, ql.y, ql.z, hl);
+            ConstU32* off = (ConstU32*)(c.cellOff);
+            const int gxgy = scl.gx * scl.gy;
+            float best = FLT_MAX;
+            uint32_t bestAt = UINT32_MAX;
+            for (int x = b.x0; x <= b.x1; ++x)
+            {
+                for (int y = b.y0; y <= b.y1; ++y)
                 {
-                    closest = d2;
-                    bestSlot = slot;
+                    for (int z = b.z0; z <= b.z1; ++z)
+                    {
+                        const uint32_t cell = scl.cellBase + static_cast<uint32_t>(x + y * scl.gx + z * gxgy);
+                        const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
+                        const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+                        for (uint32_t j0 = lo; j0 < hi; j0 += 64U)
+                        {
+                            const uint32_t j = j0 + lane;
+                            const bool in = j < hi;
+                            const F4 o = c.s0[in ? j : hi - 1U];
+                            const float ox = o.x - ql.x;
+                            const float oy = o.y - ql.y;
+                            const float oz = o.z - ql.z;
+                            const float d2 = dot3w(ox, oy, oz, ox, oy, oz);
+                            bool adm = in && d2 < hl2;
+                            if (adm)
+                            {
+                                const F4 hm = c.shm[j];
+                                const float hmin = gmin(hl, hm.x);
+                                adm = d2 < hmin * hmin && hm.y < limitl && j != tl;
+                            }
+                            uint64_t m = __ballot(adm ? 1 : 0);
+                            while (m != 0ULL)
+                            {
+                                const uint32_t a = static_cast<uint32_t>(__builtin_ctzll(m));
+                                m &= m - 1ULL;
+                                const float da = laneValue(d2, a);
+                                if (da < best)
+                                {
+                                    best = da;
+                                    bestAt = j0 + a;
+                                }
+                            }
+                        }
+                    }
                 }
-            });
+            }
+            if (lane == static_cast<uint32_t>(l))
+            {
+                closest = best;
+                bestSlot = bestAt;
+            }
         }
-        c.cand[st.denseOff + id] = bestSlot == UINT32_MAX ? kNoCandidate : c.sortedVals[bestSlot] - sc.pOff;
+        if (active)
+        {
+            c.cand[st.denseOff + id] = bestSlot == UINT32_MAX ? kNoCandidate : c.sortedVals[bestSlot] - sc.pOff;
+        }
     }
 }
 
