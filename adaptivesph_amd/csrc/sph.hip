@@ -567,6 +567,16 @@ __device__ __forceinline__ int waveMaxI(int v)
 }
 
 
+// Orders a wave's LDS writes before its lanes' later reads of them (the
+// lanes of a wave execute together; this keeps the compiler from moving
+// accesses across).
+__device__ __forceinline__ void waveSync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Read-only view in the constant address space: loads with wave-uniform
 // addresses then go through the scalar cache (the array is not written by the
 // kernels that read it this way).
@@ -816,10 +826,10 @@ __device__ __forceinline__ float laneValue(float v, uint32_t l)
 // order; q and b wave-uniform), for particles whose neighbours overflow their
 // list: a cell's slots are taken 64 at a time, one per lane; each lane whose
 // candidate is accepted (d^2 <= (2 h_j)^2) evaluates term(j, d2), then
-// fold(l) runs on every lane for the accepting lanes l in ascending order,
-// which is the visiting order. A sum folded as acc = acc + (lane l's term) is
-// the serial walk's sum bit for bit; the terms, the expensive part, are
-// evaluated 64 at a time.
+// fold(m) runs on every lane with the mask m of the accepting lanes, which it
+// folds in ascending lane order, the visiting order. A sum folded as
+// acc = acc + (lane l's term) is the serial walk's sum bit for bit; the
+// terms, the expensive part, are evaluated 64 at a time.
 template <class T, class F>
 __device__ __forceinline__ void waveWalkFold(const Ctx& c, const SceneDev& sc, uint32_t lost, const CellBox& b,
                                              const F4& q, uint32_t lane, T&& term, F&& fold)
@@ -845,7 +855,7 @@ __device__ __forceinline__ void waveWalkFold(const Ctx& c, const SceneDev& sc, u
                     const float dz = o.z - q.z;
                     const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
                     const bool acc = in && d2 <= o.w;
-                    uint64_t m = __ballot(acc ? 1 : 0);
+                    const uint64_t m = __ballot(acc ? 1 : 0);
                     if (m == 0ULL)
                     {
                         continue;
@@ -854,11 +864,7 @@ __device__ __forceinline__ void waveWalkFold(const Ctx& c, const SceneDev& sc, u
                     {
                         term(j, d2);
                     }
-                    while (m != 0ULL)
-                    {
-                        fold(static_cast<uint32_t>(__builtin_ctzll(m)));
-                        m &= m - 1ULL;
-                    }
+                    fold(m);
                 }
             }
         }
@@ -1285,7 +1291,13 @@ __global__ __launch_bounds__(256) void kDensityOverflow(Ctx c)
                 const F4 hm = c.shm[j];
                 w = hm.y * wendlandC(sqrtf(d2), hm.x, hm.z);
             },
-            [&](uint32_t l) { rho = rho + laneValue(w, l); });
+            [&](uint64_t m) {
+                while (m != 0ULL)
+                {
+                    rho = rho + laneValue(w, static_cast<uint32_t>(__builtin_ctzll(m)));
+                    m &= m - 1ULL;
+                }
+            });
         if (lane == 0U)
         {
             finishDensity(c, sc, b, q, hi, t, rho);
@@ -1905,11 +1917,19 @@ __global__ __launch_bounds__(256) void kBoundaryForce(Ctx c)
 }
 
 // Forces of the particles whose neighbours overflow their list: the walk
-// again, one wave per particle (waveWalkFold), in the same order.
+// again, one wave per particle (waveWalkFold), in the same order. The twelve
+// accumulators are folded by twelve lanes at once: each accepting lane writes
+// its pair terms to a row of the wave's LDS table (the other lanes write
+// zeros, which leave a sum unchanged), and lane k adds column k row by row.
 __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
 {
+    constexpr uint32_t kTerms = sizeof(PairAcc) / sizeof(float);
+    static_assert(kTerms == 12U, "PairAcc holds twelve accumulators");
+    __shared__ float4 terms[4U * 64U * kTerms / 4U];  // (float4: 16-byte rows of four terms)
     const uint32_t m = *c.overflowCount;
     const uint32_t lane = threadIdx.x & 63U;
+    float* table = reinterpret_cast<float*>(terms) + (threadIdx.x / 64U) * 64U * kTerms;
+    const float* column = table + (lane < kTerms ? lane : kTerms - 1U);
     const uint32_t waves = gridDim.x * (blockDim.x / 64U);
     for (uint32_t k = blockIdx.x * (blockDim.x / 64U) + threadIdx.x / 64U; k < m; k += waves)
     {
@@ -1930,8 +1950,8 @@ __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
         // each lane's pair terms: forcePair on zero accumulators gives -t for
         // the terms it subtracts and t for those it adds, and acc + (-t) is
         // acc - t exactly (the accumulators start at +0 and never become -0)
-        PairAcc A {};
         PairAcc Z {};
+        float sum = 0.0F;  // accumulator `lane` of PairAcc (lanes 0-11)
         waveWalkFold(
             c, sc, st.lostEnd, b, q, lane,
             [&](uint32_t j, float) {
@@ -1951,29 +1971,33 @@ __global__ __launch_bounds__(256) void kForceOverflow(Ctx c)
                     }
                 }
             },
-            [&](uint32_t l) {
-                A.apx = A.apx + laneValue(Z.apx, l);
-                A.apy = A.apy + laneValue(Z.apy, l);
-                A.apz = A.apz + laneValue(Z.apz, l);
-                if (doVisc)
+            [&](uint64_t mask) {
+                const bool mine = ((mask >> lane) & 1ULL) != 0ULL;
+                const float* z = reinterpret_cast<const float*>(&Z);
+                float* row = table + lane * kTerms;
+                waveSync();  // the previous step's readers are done
+#pragma unroll
+                for (uint32_t k = 0; k < kTerms; k += 4U)
                 {
-                    A.avx = A.avx + laneValue(Z.avx, l);
-                    A.avy = A.avy + laneValue(Z.avy, l);
-                    A.avz = A.avz + laneValue(Z.avz, l);
+                    *reinterpret_cast<float4*>(row + k) = mine ? float4 {z[k], z[k + 1U], z[k + 2U], z[k + 3U]}
+                                                               : float4 {0.0F, 0.0F, 0.0F, 0.0F};
                 }
-                if (doST)
+                waveSync();
+                const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(mask));
+                const uint32_t last = 63U - static_cast<uint32_t>(__builtin_clzll(mask));
+#pragma unroll 8
+                for (uint32_t a = first; a <= last; ++a)
                 {
-                    A.asx = A.asx + laneValue(Z.asx, l);
-                    A.asy = A.asy + laneValue(Z.asy, l);
-                    A.asz = A.asz + laneValue(Z.asz, l);
-                }
-                if (nuLam != 0.0F)
-                {
-                    A.amx = A.amx + laneValue(Z.amx, l);
-                    A.amy = A.amy + laneValue(Z.amy, l);
-                    A.amz = A.amz + laneValue(Z.amz, l);
+                    sum = sum + column[a * kTerms];
                 }
             });
+        PairAcc A;
+        float* acc = reinterpret_cast<float*>(&A);
+#pragma unroll
+        for (uint32_t k = 0; k < kTerms; ++k)
+        {
+            acc[k] = laneValue(sum, k);
+        }
         if (lane == 0U)
         {
             finishForce(c, sc, st, b, q, vi, ri, epsH2, doVisc, doST, nuLam, t, A);
