@@ -1203,30 +1203,42 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         const float hi = c.shm[t].x;
         const float R = 2.0F * hi;
         const CellBox b = cellBox(sc, q.x, q.y, q.z, R);
-        // entries [done, cnt) are staged at ring positions (done .. cnt-1) mod
-        // kRing; at most kRing - 1 are staged, so position cnt mod kRing is free
-        uint32_t cnt = 0;
+        // Entries [done, cnt) are staged at ring positions (done .. cnt-1) mod
+        // kRing; at most kRing - 1 are staged, so position cnt mod kRing is
+        // free. The counters are kept as byte offsets into the ring: entry k
+        // of the thread sits at (k mod kRing) * kRow + 4 tid, kRow = 4 BS, and
+        // cntA = cnt * kRow + 4 tid (mod 2^32, a multiple of the ring's size),
+        // so the next entry's offset is one mask and a hit one add. `done`
+        // is also kept as a plain count, which gives cnt at the end.
+        constexpr uint32_t kRow = 4U * BS;
+        constexpr uint32_t kRingMask = kRing * kRow - 1U;
+        uint32_t cntA = 4U * tid;
+        uint32_t doneA = 4U * tid;
         uint32_t done = 0;
+        char* ring = reinterpret_cast<char*>(stage);
+        auto staged = [&](uint32_t off) { return *reinterpret_cast<const uint32_t*>(ring + off); };
         // the thread's next chunk (listChunk): chunk rows of a wave's block
         // are 64 chunks apart
         uint4* cptr = reinterpret_cast<uint4*>(c.list) + listChunk(c, pq, 0U);
         constexpr uint32_t cstep = 64U;
         auto push = [&](bool hit, uint32_t j) {
-            stage[(cnt & (kRing - 1U)) * BS + tid] = j;
-            cnt += hit ? 1U : 0U;
+            *reinterpret_cast<uint32_t*>(ring + (cntA & kRingMask)) = j;
+            cntA += hit ? kRow : 0U;
         };
         // writes the oldest staged chunk once it is full (called after at most
         // kChunk pushes, so the ring never holds more than kRing - 1 entries)
         auto drain = [&]() {
-            if (cnt - done >= kChunk)
+            if (cntA - doneA >= kChunk * kRow)
             {
                 if (done < K)
                 {
-                    const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                    storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
+                    // done is a multiple of kChunk: the chunk does not wrap
+                    const uint32_t e0 = doneA & kRingMask;
+                    storeChunk(cptr, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow), staged(e0 + 3U * kRow));
                     cptr += cstep;
                 }
                 done += kChunk;
+                doneA += kChunk * kRow;
             }
         };
         if (oneScene)
@@ -1240,13 +1252,14 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
                 drain();
             });
         }
+        const uint32_t cnt = done + (cntA - doneA) / kRow;
         if (valid)
         {
             if (cnt <= K && cnt > done)
             {
                 // partial last chunk (entries past cnt are never read)
-                const uint32_t e0 = (done & (kRing - 1U)) * BS + tid;
-                storeChunk(cptr, stage[e0], stage[e0 + BS], stage[e0 + 2U * BS], stage[e0 + 3U * BS]);
+                const uint32_t e0 = doneA & kRingMask;
+                storeChunk(cptr, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow), staged(e0 + 3U * kRow));
             }
             c.ncount[t] = cnt;
             if (cnt > K)
