@@ -1167,6 +1167,13 @@ __device__ __forceinline__ float listDensityFast(const Ctx& c, const SceneDev& s
 // thread builds its particle's list, then sums the density over it. BS (the
 // block size) is a compile-time constant so that the staging addresses fold
 // into immediates.
+// Most cells in a wave's union box for the wave-uniform walk (a compact wave
+// of 64 queries spans 2-3 cells per axis).
+#ifndef ASPH_MAX_UNION_CELLS
+#define ASPH_MAX_UNION_CELLS 64
+#endif
+constexpr uint32_t kMaxUnionCells = ASPH_MAX_UNION_CELLS;
+
 template <uint32_t BS>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
 {
@@ -1241,7 +1248,18 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
                 doneA += kChunk * kRow;
             }
         };
-        if (oneScene)
+        // A wave whose queries lie far apart (spray, a sparse surface) would
+        // walk every cell of its union box: its lanes walk their own cells
+        // instead, as do lanes of waves that straddle scenes.
+        const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
+        const int uy0 = waveMinI(valid ? b.y0 : 0x7FFFFFFF);
+        const int uz0 = waveMinI(valid ? b.z0 : 0x7FFFFFFF);
+        const int ux1 = waveMaxI(valid ? b.x1 : -1);
+        const int uy1 = waveMaxI(valid ? b.y1 : -1);
+        const int uz1 = waveMaxI(valid ? b.z1 : -1);
+        const float unionCells = static_cast<float>(ux1 - ux0 + 1) * static_cast<float>(uy1 - uy0 + 1) *
+                                 static_cast<float>(uz1 - uz0 + 1);
+        if (oneScene && unionCells <= static_cast<float>(kMaxUnionCells))
         {
             waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain);
         }

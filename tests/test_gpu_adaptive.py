@@ -202,6 +202,37 @@ def test_cfg3_sloshing_tank_at_full_size():
     ref.close()
 
 
+def test_split_heavy_tank_at_scale_meets_the_cap():
+    """The 500,000-particle tank with a refinement zone covering all of it and
+    a cap 4,000 above the start: each refinement step splits up to the count
+    its merges freed, rounded up to whole blocks (a net gain of 1,000-2,000
+    particles a step), until the capacity clamp (AdaptiveSphSimulation.cu:
+    178-190) takes over at scale; decisions and state equal the FP32 oracle's
+    for 4 steps, and the count ends within one block of splits of the cap."""
+    s = scene.cfg3()
+    cap = s.count + 4_000
+    s.refinement.max_particle_count = cap
+    s.capacity = cap
+    s.refinement.zone = abi.ZoneParameters(abi.Vec3(0.5, 0.2, 0.32), 2.0, 0.0, 2.0)
+    sim = Simulation.from_scene(s)
+    ref = oracle.OracleSim.from_scene(s)
+    split = 0
+    counts = []
+    for step in range(1, 5):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        ta, tb = sim.refinement_trace(), ref.refinement_trace()
+        _compare_traces(ta, tb, f"split-heavy step {step}")
+        split += ta["split_performed"]
+        counts.append(sim.get_fluid_particles_count())
+        assert counts[-1] == ref.count() <= cap
+    assert_bitexact(sim.fluid(), ref.fluid(), "split-heavy step 4")
+    print(f"splits {split}, counts {counts}, cap {cap}")
+    assert split > 1000 and cap - counts[-1] < 12 * 256, (split, counts, cap)
+    sim.close()
+    ref.close()
+
+
 def test_cfg3_reference_merged_h_at_full_size():
     """The 500,000-particle sloshing tank with the reference's merged
     smoothing length (flags 0, AdaptiveAlgorithm.cu:50-67): 4 refinement
