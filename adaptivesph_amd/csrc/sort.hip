@@ -24,7 +24,10 @@ namespace
 {
 
 constexpr int kBlock = 256;
-constexpr int kItems = 16;
+#ifndef ASPH_SORT_ITEMS
+#define ASPH_SORT_ITEMS 8
+#endif
+constexpr int kItems = ASPH_SORT_ITEMS;  // elements per thread of a tile
 constexpr int kTile = kBlock * kItems;
 constexpr int kWaves = kBlock / 64;
 
