@@ -1167,10 +1167,17 @@ __device__ __forceinline__ float listDensityFast(const Ctx& c, const SceneDev& s
 // thread builds its particle's list, then sums the density over it. BS (the
 // block size) is a compile-time constant so that the staging addresses fold
 // into immediates.
-// Most cells in a wave's union box for the wave-uniform walk (a compact wave
-// of 64 queries spans 2-3 cells per axis).
+// Most cells in a wave's union box for the wave-uniform walk. A compact wave
+// of 64 queries spans 2-3 cells per axis (27-54 cells); a wave of the sparse
+// layer that forms under a falling block or along a free surface (about one
+// particle per cell along a row) spans a few hundred mostly empty cells and
+// still walks faster as a wave than lane by lane, whose 27-cell chains of
+// dependent loads make such a wave the kernel's last to finish; only
+// scattered spray exceeds this. (1M dam break, ms/step over 1,000 steps, by
+// limit: 64: 1.080, 256: 1.076, 384: 1.047, 512: 1.051, 1024: 1.060,
+// unlimited: 1.131.)
 #ifndef ASPH_MAX_UNION_CELLS
-#define ASPH_MAX_UNION_CELLS 64
+#define ASPH_MAX_UNION_CELLS 384
 #endif
 constexpr uint32_t kMaxUnionCells = ASPH_MAX_UNION_CELLS;
 
