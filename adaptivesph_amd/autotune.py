@@ -8,8 +8,10 @@ Five launch parameters shape a step, each acting on its own stages:
                  width with the fewest passes);
 * block_neighbor threads per block of the neighbour kernels (density, forces);
 * query_tile     the cell-tile shape of the neighbour kernels' waves: the
-                 sorted (cell) order, or 2x2x2 / 4x4x4 sub-cell tiles within
-                 each cell (the reordering runs in the gather stage);
+                 sorted (cell) order, 2x2x2 / 4x4x4 sub-cell tiles in Morton
+                 order, 4x4x4 sub-cells in x-major order (slabs) within each
+                 cell, or slabs per scene once its waves loosen (the
+                 reordering runs in the gather stage);
 * force_blocks   the force kernels' grid: one thread per particle, or a
                  persistent grid of 4, 8 or 16 blocks per compute unit.
 
@@ -31,7 +33,7 @@ SPACE = {
     "block_stream": ((128, 256, 512, 1024), ("kick_drift_key", "gather")),
     "radix_bits": ((0, 4, 6, 7, 8, 10), ("cell_sort",)),
     "block_neighbor": ((64, 128, 256), ("neighbors", "density", "force")),
-    "query_tile": ((0, 1, 2), ("gather", "neighbors", "density", "force")),
+    "query_tile": ((0, 1, 2, 3, 4), ("gather", "neighbors", "density", "force")),
     "force_blocks": ((0, 4, 8, 16), ("force",)),
 }
 # query_tile -1: the engine's own default (sub-cell tiles in adaptive batches)

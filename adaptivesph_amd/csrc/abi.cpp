@@ -792,9 +792,9 @@ int asph_batch_set_schedule(asph_batch* batch, int32_t query_tile, uint32_t forc
 {
     return guard([&] {
         asph::Engine& e = engineOf(batch);
-        if (query_tile < -1 || query_tile > 2)
+        if (query_tile < -1 || query_tile > 4)
         {
-            throw asph::Error(ASPH_ERR_INVALID, "query_tile must be -1 (default), 0, 1 or 2");
+            throw asph::Error(ASPH_ERR_INVALID, "query_tile must be -1 (default), 0, 1, 2, 3 or 4");
         }
         if (force_blocks_per_cu > 64U)
         {

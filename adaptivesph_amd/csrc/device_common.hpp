@@ -25,6 +25,14 @@ constexpr uint32_t kWave = 64;
 constexpr uint32_t kGroup = ASPH_GROUP;  // sorted slots per culling box (a power of two, at most 64)
 constexpr uint32_t kBoundaryGroup = 16;  // boundary slots per culling box (chunks of a cell's range)
 
+// Query order modes (Ctx::queryTileBits, with qperm): 1 and 2 Morton 2x2x2 /
+// 4x4x4 sub-cells, 3 x-major 4x4x4 slabs, kQueryAuto: x-major slabs for the
+// scenes whose waves have loosened (SceneState::qorder, set by kPrep from the
+// list builder's counts), the sorted order for the others.
+constexpr uint32_t kQueryAuto = 4;
+constexpr uint32_t kQuerySample = 64;       // one wave in 64 counts for the automatic order
+constexpr uint32_t kQueryMinHits = 65536;   // neighbours counted before a decision (~16 waves)
+
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;
 constexpr uint32_t kSceneAdaptive = 2U;
@@ -86,7 +94,12 @@ struct SceneState
     uint32_t splitDone;
     uint32_t countAfter;
     uint32_t refined;
-    uint32_t pad;
+    // automatic query order (query tile mode 4): on once this scene's waves
+    // have loosened (slabbed queries from then on); the list builder's
+    // survivor and neighbour counts of the last step decide
+    uint32_t qorder;
+    uint32_t qsurv;
+    uint32_t qhits;
 };
 
 struct F4

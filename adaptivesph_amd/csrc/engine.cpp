@@ -325,6 +325,7 @@ Engine::Engine(const EngineConfig& cfg)
         c.overflowCount = alloc<uint32_t>(1);
         c.wallQueue = alloc<uint32_t>(P);
         c.wallQueueCount = alloc<uint32_t>(1);
+        c.queryAny = alloc<uint32_t>(1);
         cfg_.listCap = listCapacityRounded(cfg_.listCap);
         c.listCap = cfg_.listCap;
         // wave-blocked rows: whole waves of 64 slots
@@ -379,6 +380,7 @@ Engine::Engine(const EngineConfig& cfg)
         hipCheck(hipMemset(c.bcellOff, 0, sizeof(uint32_t) * (C + 1)), "hipMemset");
         hipCheck(hipMemset(nOrderDev_, 0, sizeof(uint32_t)), "hipMemset");
         hipCheck(hipMemset(c.nTotal, 0, sizeof(uint32_t)), "hipMemset");
+        hipCheck(hipMemset(c.queryAny, 0, sizeof(uint32_t)), "hipMemset");
         if (cfg_.adaptive)
         {
             hipCheck(hipMemset(c.removal, 0, sizeof(uint32_t) * P), "hipMemset");
@@ -446,7 +448,8 @@ void Engine::invalidateGraphs()
 void Engine::buildCtx()
 {
     Ctx& c = ctx_;
-    const int tile = cfg_.launch.queryTile < 0 ? (cfg_.adaptive ? 2 : 0) : cfg_.launch.queryTile;
+    const int tile = cfg_.launch.queryTile < 0 ? (cfg_.adaptive ? 2 : static_cast<int>(kQueryAuto))
+                                               : cfg_.launch.queryTile;
     c.qperm = tile > 0 ? qpermBuf_ : nullptr;
     c.qpos = tile > 0 ? qposBuf_ : nullptr;
     c.queryTileBits = static_cast<uint32_t>(tile);

@@ -89,6 +89,7 @@ struct Ctx
     // slots; walls within reach), and their number (reset by kPrep each step)
     uint32_t* wallQueue;
     uint32_t* wallQueueCount;
+    uint32_t* queryAny;  // device: some scene takes the query order this step (kPrep)
     uint32_t* list;  // [listCap * pCap rounded up to 64], 4-entry chunks, wave-blocked: entry k of slot t
                      // at (((t / 64) * (listCap / 4) + k / 4) * 64 + t % 64) * 4 + k % 4
     uint32_t listCap;  // a multiple of 4
@@ -131,7 +132,7 @@ struct Ctx
     // cell with daughters in index order.
     uint32_t* qperm;
     uint32_t* qpos;
-    uint32_t queryTileBits;  // sub-cell bits per axis of the query order (1 or 2; with qperm)
+    uint32_t queryTileBits;  // query order mode (with qperm): 1, 2, 3 or kQueryAuto
     uint32_t* removal;
     uint32_t* keep;
     uint32_t* rprefix;

@@ -81,6 +81,13 @@ __device__ __forceinline__ bool mergeParticipant(const SceneDev& sc, const Scene
 __global__ __launch_bounds__(256) void kPrep(Ctx c)
 {
     __shared__ uint32_t lds[4];
+    __shared__ uint32_t anyQuery;
+    if (threadIdx.x == 0)
+    {
+        anyQuery = 0U;
+    }
+    __syncthreads();
+    uint32_t anyOrdered = 0;
     uint32_t carry = 0;
     for (uint32_t s0 = 0; s0 < c.sceneSlots; s0 += 256)
     {
@@ -137,6 +144,23 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
                 }
                 st.refineNow = refine;
                 st.time = t;
+                if (c.queryTileBits == kQueryAuto && st.qorder == 0U && st.qhits >= kQueryMinHits)
+                {
+                    // slabbed queries once the sampled waves kept more than
+                    // 8.5 survivors per neighbour of a particle (a freshly
+                    // generated lattice: about 6.5; a mixed dam break: 14);
+                    // the counts accumulate over steps until they cover
+                    // enough waves
+                    const uint64_t surv = st.qsurv;
+                    const uint64_t hits = st.qhits;
+                    st.qorder = surv * 128U > hits * 17U ? 1U : 0U;
+                    st.qsurv = 0U;
+                    st.qhits = 0U;
+                }
+                if (st.qorder != 0U || c.queryTileBits != kQueryAuto)
+                {
+                    anyOrdered = 1U;
+                }
                 st.positives = 0;
                 st.mergePairs = 0;
                 if (refine != 0U)
@@ -178,8 +202,14 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
         }
         mcarry += tot;
     }
+    if (anyOrdered != 0U)
+    {
+        atomicOr(&anyQuery, 1U);
+    }
+    __syncthreads();
     if (threadIdx.x == 0)
     {
+        *c.queryAny = anyQuery;
         *c.nTotal = carry;
         *c.overflowCount = 0U;
         *c.wallQueueCount = 0U;
@@ -363,85 +393,6 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
     }
 }
 
-// Query order (LaunchCfg::queryTile): within each cell of more than 64
-// slots, the slots grouped by their sub-cell (c.queryTileBits bits per axis:
-// 2x2x2 or 4x4x4 sub-cells, in Morton order), one wave per cell. Any grouping
-// gives the same lists and sums (each lane's list depends on its own query
-// only); grouping keeps a wave's queries close together, so its candidate
-// filter keeps few survivors even in refined cells, whose particles are in
-// index order (daughters appended per split).
-__global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
-{
-    __shared__ uint32_t bins[4][64];
-    const uint32_t lane = threadIdx.x & 63U;
-    const uint32_t w = threadIdx.x >> 6U;
-    const uint32_t waves = gridDim.x * 4U;
-    // cellOff holds entries 0 .. cExtent - 1 (the last one an end entry):
-    // the ranges of cells 0 .. cExtent - 2 cover every sorted slot once
-    for (uint32_t cell = blockIdx.x * 4U + w; cell + 1U < c.cExtent; cell += waves)
-    {
-        const uint32_t lo = c.cellOff[cell];
-        const uint32_t hi = c.cellOff[cell + 1U];
-        if (hi - lo <= 64U)
-        {
-            // one wave's worth: the sorted order
-            if (lane < hi - lo)
-            {
-                c.qperm[lo + lane] = lo + lane;
-                c.qpos[lo + lane] = lo + lane;
-            }
-            continue;
-        }
-        uint32_t* bin = bins[w];
-        bin[lane] = 0U;
-        const SceneDev& sc = c.scenes[c.sscene[lo]];
-        const uint32_t bits = c.queryTileBits;  // 1 or 2
-        const uint32_t side = 1U << bits;
-        auto subOf = [&](uint32_t t) {
-            const F4 q = c.s0[t];
-            const float u[3] = {(q.x - sc.dmin[0]) / sc.cs[0], (q.y - sc.dmin[1]) / sc.cs[1],
-                                (q.z - sc.dmin[2]) / sc.cs[2]};
-            uint32_t m = 0;
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-            {
-                const float f = u[a] - floorf(u[a]);
-                const uint32_t k = min(static_cast<uint32_t>(fmaxf(f, 0.0F) * static_cast<float>(side)), side - 1U);
-                // Morton: bit b of axis a at 3b + a (below 64 bins)
-                m |= ((k & 1U) << a) | (((k >> 1U) & 1U) << (a + 3));
-            }
-            return m;
-        };
-        for (uint32_t t0 = lo; t0 < hi; t0 += 64U)
-        {
-            const uint32_t t = t0 + lane;
-            if (t < hi)
-            {
-                atomicAdd(&bin[subOf(t)], 1U);
-            }
-        }
-        // exclusive scan of the 64 bins across the wave
-        const uint32_t count = bin[lane];
-        uint32_t incl = count;
-        for (uint32_t off = 1; off < 64U; off <<= 1U)
-        {
-            const uint32_t y = __shfl_up(incl, off, 64);
-            incl += lane >= off ? y : 0U;
-        }
-        bin[lane] = lo + incl - count;
-        for (uint32_t t0 = lo; t0 < hi; t0 += 64U)
-        {
-            const uint32_t t = t0 + lane;
-            if (t < hi)
-            {
-                const uint32_t p = atomicAdd(&bin[subOf(t)], 1U);
-                c.qperm[p] = t;
-                c.qpos[t] = p;
-            }
-        }
-    }
-}
-
 // True unless every member of the group is provably farther from q than its
 // own support 2h (conservative by a relative margin far above FP32 rounding).
 __device__ __forceinline__ bool groupMayHit(const F4& b0, const F4& b1, const F4& q)
@@ -575,6 +526,173 @@ __device__ __forceinline__ void waveSync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool queryOrdered(const Ctx& c, uint32_t s)
+{
+    return c.qperm != nullptr && (c.queryTileBits != kQueryAuto || c.states[s].qorder != 0U);
+}
+
+// The sorted slot whose neighbour list sits at query position pq (the query
+// order permutes slots within cells, so pq and the slot share a scene).
+__device__ __forceinline__ uint32_t querySlot(const Ctx& c, uint32_t pq)
+{
+    if (c.qperm == nullptr || (c.queryTileBits == kQueryAuto && c.states[c.sscene[pq]].qorder == 0U))
+    {
+        return pq;
+    }
+    return c.qperm[pq];
+}
+
+// Query order (LaunchCfg::queryTile): within each cell of more than 64
+// slots, the slots grouped by their sub-cell, one wave per cell: modes 1 and
+// 2 take 2x2x2 or 4x4x4 sub-cells in Morton order, mode 3 4x4x4 sub-cells in
+// x-major order (x, then y, then z), so that any 64 consecutive queries form
+// a slab of the cell across x and, as the next cell in key order is the next
+// along x, a wave that runs into the next cell continues the same slab (the
+// order a lattice generated with x slowest has in index order).
+// Any grouping gives the same lists and sums (each lane's list depends on its
+// own query only); grouping keeps a wave's queries close together, so its
+// candidate filter keeps few survivors even in refined cells, whose particles
+// are in index order (daughters appended per split), and in mixed fluid,
+// where a cell's particles in index order lie all over it.
+
+// The query order of one cell of more than 64 slots [lo, hi): a counting
+// sort of its slots by sub-cell (64 bins in LDS), ties in any order.
+__device__ __forceinline__ void queryOrderCell(const Ctx& c, uint32_t lo, uint32_t hi, uint32_t lane, uint32_t* bin)
+{
+    bin[lane] = 0U;
+    const SceneDev& sc = c.scenes[c.sscene[lo]];
+    const uint32_t mode = c.queryTileBits == kQueryAuto ? 3U : c.queryTileBits;  // 1, 2 or 3
+    const uint32_t side = mode == 1U ? 2U : 4U;
+    // the position within its cell in units of sub-cells (only the grouping
+    // depends on it, so reciprocals serve)
+    const float sx = static_cast<float>(side) / sc.cs[0];
+    const float sy = static_cast<float>(side) / sc.cs[1];
+    const float sz = static_cast<float>(side) / sc.cs[2];
+    auto subOf = [&](const F4& q) {
+        const float u[3] = {(q.x - sc.dmin[0]) * sx, (q.y - sc.dmin[1]) * sy, (q.z - sc.dmin[2]) * sz};
+        uint32_t k[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            const float f = u[a] - static_cast<float>(side) * floorf(u[a] / static_cast<float>(side));
+            k[a] = min(static_cast<uint32_t>(fmaxf(f, 0.0F)), side - 1U);
+        }
+        if (mode == 3U)
+        {
+            return (k[0] * 4U + k[1]) * 4U + k[2];
+        }
+        uint32_t m = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+        {
+            // Morton: bit b of axis a at 3b + a (below 64 bins)
+            m |= ((k[a] & 1U) << a) | (((k[a] >> 1U) & 1U) << (a + 3));
+        }
+        return m;
+    };
+    // the first kKeep rounds of 64 slots: positions loaded together, their
+    // sub-cells kept for the placement pass
+    constexpr uint32_t kKeep = 4;
+    uint32_t kept[kKeep];
+    {
+        F4 q[kKeep];
+#pragma unroll
+        for (uint32_t r = 0; r < kKeep; ++r)
+        {
+            const uint32_t t = lo + r * 64U + lane;
+            q[r] = c.s0[t < hi ? t : lo];
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < kKeep; ++r)
+        {
+            kept[r] = subOf(q[r]);
+        }
+    }
+    waveSync();
+#pragma unroll
+    for (uint32_t r = 0; r < kKeep; ++r)
+    {
+        if (lo + r * 64U + lane < hi)
+        {
+            atomicAdd(&bin[kept[r]], 1U);
+        }
+    }
+    for (uint32_t t0 = lo + kKeep * 64U; t0 < hi; t0 += 64U)
+    {
+        const uint32_t t = t0 + lane;
+        if (t < hi)
+        {
+            atomicAdd(&bin[subOf(c.s0[t])], 1U);
+        }
+    }
+    waveSync();
+    // exclusive scan of the 64 bins across the wave
+    const uint32_t count = bin[lane];
+    uint32_t incl = count;
+    for (uint32_t off = 1; off < 64U; off <<= 1U)
+    {
+        const uint32_t y = __shfl_up(incl, off, 64);
+        incl += lane >= off ? y : 0U;
+    }
+    bin[lane] = lo + incl - count;
+    waveSync();
+    for (uint32_t t0 = lo, r = 0; t0 < hi; t0 += 64U, ++r)
+    {
+        const uint32_t t = t0 + lane;
+        if (t < hi)
+        {
+            uint32_t sub = 0U;
+#pragma unroll
+            for (uint32_t rr = 0; rr < kKeep; ++rr)
+            {
+                sub = r == rr ? kept[rr] : sub;
+            }
+            if (r >= kKeep)
+            {
+                sub = subOf(c.s0[t]);
+            }
+            const uint32_t p = atomicAdd(&bin[sub], 1U);
+            c.qperm[p] = t;
+            c.qpos[t] = p;
+        }
+    }
+    waveSync();
+}
+
+__global__ __launch_bounds__(256) void kQueryOrder(Ctx c)
+{
+    __shared__ uint32_t bins[4][64];
+    const uint32_t lane = threadIdx.x & 63U;
+    const uint32_t w = threadIdx.x >> 6U;
+    const uint32_t waves = gridDim.x * 4U;
+    if (*c.queryAny == 0U)
+    {
+        return;  // every scene keeps the sorted order this step
+    }
+    // cellOff holds entries 0 .. cExtent - 1 (the last one an end entry):
+    // the ranges of cells 0 .. cExtent - 2 cover every sorted slot once
+    for (uint32_t cell = blockIdx.x * 4U + w; cell + 1U < c.cExtent; cell += waves)
+    {
+        const uint32_t lo = c.cellOff[cell];
+        const uint32_t hi = c.cellOff[cell + 1U];
+        if (hi <= lo || !queryOrdered(c, c.sscene[lo]))
+        {
+            continue;  // empty, or a scene that keeps the sorted order
+        }
+        if (hi - lo <= 64U)
+        {
+            // one wave's worth: the sorted order
+            if (lane < hi - lo)
+            {
+                c.qperm[lo + lane] = lo + lane;
+                c.qpos[lo + lane] = lo + lane;
+            }
+            continue;
+        }
+        queryOrderCell(c, lo, hi, lane, bins[w]);
+    }
 }
 
 // Read-only view in the constant address space: loads with wave-uniform
@@ -731,7 +849,7 @@ constexpr uint32_t kStashPad = 4;  // table entries past 64 read (and ignored) b
 template <class P, class D>
 __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid,
                                                  const CellBox& b, const F4& q, F4* wstash, uint32_t* wslot,
-                                                 uint32_t lane, P&& push, D&& drain)
+                                                 uint32_t lane, P&& push, D&& drain, uint32_t& survivors)
 {
     ConstU32* off = (ConstU32*)(c.cellOff);
     const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
@@ -780,6 +898,7 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
                         continue;
                     }
                     const uint32_t S = static_cast<uint32_t>(__builtin_popcountll(m));
+                    survivors += S;
                     if (keep)
                     {
                         // rank among the survivors = survivors in lower lanes
@@ -1207,7 +1326,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         const bool valid = tw < n;
         // query position pq handles sorted slot t; the list sits at pq
         const uint32_t pq = valid ? tw : n - 1U;
-        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
+        const uint32_t t = querySlot(c, pq);
         const uint32_t s = c.sscene[t];
         const uint32_t sw = __builtin_amdgcn_readfirstlane(s);
         const bool oneScene = __ballot((valid && s != sw) ? 1 : 0) == 0ULL;
@@ -1266,9 +1385,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         const int uz1 = waveMaxI(valid ? b.z1 : -1);
         const float unionCells = static_cast<float>(ux1 - ux0 + 1) * static_cast<float>(uy1 - uy0 + 1) *
                                  static_cast<float>(uz1 - uz0 + 1);
-        if (oneScene && unionCells <= static_cast<float>(kMaxUnionCells))
+        const bool waveWalk = oneScene && unionCells <= static_cast<float>(kMaxUnionCells);
+        uint32_t survivors = 0;
+        if (waveWalk)
         {
-            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain);
+            waveWalkFiltered(c, c.scenes[sw], c.states[sw].lostEnd, valid, b, q, wstash, wslot, lane, push, drain,
+                             survivors);
         }
         else if (valid)
         {
@@ -1278,6 +1400,23 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
             });
         }
         const uint32_t cnt = done + (cntA - doneA) / kRow;
+        if (waveWalk && c.queryTileBits == kQueryAuto && (blockIdx.x * (BS / 64U) + tid / 64U) % kQuerySample == 0U &&
+            c.states[sw].qorder == 0U)
+        {
+            // the automatic query order's measure, on one wave in kQuerySample
+            // (atomics of every wave on one address would serialise):
+            // survivors per wave against neighbours per particle
+            uint32_t hits = valid ? cnt : 0U;
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                hits += static_cast<uint32_t>(__shfl_xor(static_cast<int>(hits), off, 64));
+            }
+            if (lane == 0U)
+            {
+                atomicAdd(&c.states[sw].qsurv, survivors);
+                atomicAdd(&c.states[sw].qhits, hits);
+            }
+        }
         if (valid)
         {
             if (cnt <= K && cnt > done)
@@ -1836,7 +1975,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // query positions pq, in the list builder's order (slot t's list at pq)
     for (uint32_t pq = xcdBlock(n) * blockDim.x + threadIdx.x; pq < n; pq += gridDim.x * blockDim.x)
     {
-        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
+        const uint32_t t = querySlot(c, pq);
         const uint32_t cnt = c.ncount[t];
         if (cnt > K)
         {
@@ -1899,7 +2038,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ASPH_FORCE_
     const uint32_t K = c.listCap;
     for (uint32_t pq = xcdBlock(n) * blockDim.x + threadIdx.x; pq < n; pq += gridDim.x * blockDim.x)
     {
-        const uint32_t t = c.qperm != nullptr ? c.qperm[pq] : pq;
+        const uint32_t t = querySlot(c, pq);
         const uint32_t cnt = c.ncount[t];
         if (cnt > K)
         {
@@ -2404,7 +2543,7 @@ __global__ __launch_bounds__(256) void kMergeCandidates(Ctx c)
         const uint32_t cnt = active ? c.ncount[ti] : 0U;
         if (active && cnt <= c.listCap)
         {
-            const uint32_t lpi = c.qperm != nullptr ? c.qpos[ti] : ti;  // the list's query position
+            const uint32_t lpi = queryOrdered(c, c.sscene[ti]) ? c.qpos[ti] : ti;  // the list's query position
             // Every hit of the walk over the cells within h_i is in the
             // particle's neighbour list of this step (d2 < h_j^2 < (2 h_j)^2,
             // cells within h_i inside those within 2 h_i), and the list holds

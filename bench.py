@@ -148,14 +148,38 @@ def cpu_baseline(scene_obj, steps: int, runs: int) -> dict:
     }
 
 
-def stage_profile(batch, steps: int) -> dict:
-    """Per-stage device time (HIP events between the stages of eager steps on
-    the engine's stream), averaged over `steps` steps."""
-    stages = {}
-    for _ in range(max(steps, 1)):
-        for k, v in batch.profile_step().items():
-            stages.setdefault(k, []).append(v)
-    return {k: sum(v) / len(v) for k, v in stages.items()}
+def timed_steps(batch, steps: int, samples: int, count_particles: bool) -> tuple[float, int, dict]:
+    """The timed region: `steps` steps, of which `samples` (spread evenly over
+    the region) run eagerly with HIP events recorded on the engine's stream
+    between their stages, the others as graph replays. Returns the elapsed
+    wall time (device synchronised on both sides), the particles updated (each
+    step's count when count_particles, else 0) and each stage's device time
+    averaged over the sampled steps. The per-stage times therefore cover the
+    same steps as the throughput, as a step's cost drifts with the flow."""
+    samples = max(1, min(samples, steps))
+    every = steps // samples  # step k is sampled when k % every == every - 1
+    stages: dict = {}
+    taken = 0
+    particles = 0
+    batch.synchronize()
+    t0 = time.perf_counter()
+    k = 0
+    while k < steps:
+        if k % every == every - 1 and taken < samples:
+            for name, v in batch.profile_step().items():
+                stages.setdefault(name, []).append(v)
+            taken += 1
+            run = 1
+        else:
+            # graph replays up to the next sampled step
+            run = 1 if count_particles else max(1, min(every - 1 - k % every, steps - k))
+            batch.step(run)
+        if count_particles:
+            particles += batch.step_particles()
+        k += run
+    batch.synchronize()
+    elapsed = time.perf_counter() - t0
+    return elapsed, particles, {name: sum(v) / len(v) for name, v in stages.items()}
 
 
 def counters_key(workload: str, math: str) -> str:
@@ -259,23 +283,14 @@ def run_single(args) -> dict:
     log(f"workload {workload}: {s.count} particles")
     batch.step(args.warmup)
     batch.synchronize()
-    stage_ms = stage_profile(batch, args.profile_steps)
+    # the particle count of an adaptive scene changes every step: count each
+    # step's particles
+    elapsed, particles, stage_ms = timed_steps(batch, args.steps, args.profile_steps, s.adaptive)
+    if not s.adaptive:
+        particles = s.count * args.steps
     log("stage ms:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
     stage_particles = batch.step_particles()
     pairs = batch.average_neighbor_count(sid) * batch.count(sid)
-    batch.synchronize()
-    particles = 0
-    t0 = time.perf_counter()
-    if s.adaptive:
-        # the particle count changes every step: count each step's particles
-        for _ in range(args.steps):
-            batch.step(1)
-            particles += batch.step_particles()
-    else:
-        batch.step(args.steps)
-        batch.synchronize()
-        particles = s.count * args.steps
-    elapsed = time.perf_counter() - t0
     count = batch.count(sid)
     batch.close()
     value = particles / elapsed
@@ -326,15 +341,9 @@ def measure_batch(args, with_sequential: bool) -> dict:
     batch.set_schedule(tune["query_tile"], tune["force_blocks"])
     batch.step(args.warmup)
     batch.synchronize()
-    particles = 0
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        batch.step(1)
-        particles += batch.step_particles()
-    elapsed = time.perf_counter() - t1
+    elapsed, particles, stage_ms = timed_steps(batch, args.steps, args.profile_steps, True)
     log(f"batched: {elapsed / args.steps * 1e3:.2f} ms/step")
     lat = [batch.step_timed() for _ in range(args.latency_steps)]
-    stage_ms = stage_profile(batch, args.profile_steps)
     log("stage ms:", json.dumps({k: round(v, 4) for k, v in stage_ms.items()}))
     stage_particles = batch.step_particles()
     pairs = sum(batch.average_neighbor_count(i) * batch.count(i) for i in sids)
@@ -511,7 +520,8 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-sequential", action="store_true",
                     help="batch workload: leave out the per-scene sequential comparison")
-    ap.add_argument("--profile-steps", type=int, default=5)
+    ap.add_argument("--profile-steps", type=int, default=5,
+                    help="steps of the timed region (spread evenly) whose stages are timed with HIP events")
     ap.add_argument("--latency-steps", type=int, default=50)
     ap.add_argument("--rate", type=float, default=2.8, help="serve: requests per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")

@@ -99,16 +99,16 @@ def test_launch_shapes_do_not_change_results():
 
 @pytest.mark.parametrize("tolerance", [False, True])
 def test_schedules_do_not_change_results(tolerance):
-    """Query tiles (cell order, 2x2x2 and 4x4x4 sub-cell tiles) and force
-    grids (one thread per particle, persistent grids) give bit-identical
-    results, for plain and refining scenes (whose cells exceed 64 particles
-    once refined) in both arithmetic modes."""
+    """Query tiles (cell order, 2x2x2 and 4x4x4 Morton sub-cell tiles, 4x4x4
+    x-major slabs, automatic) and force grids (one thread per particle, persistent grids)
+    give bit-identical results, for plain and refining scenes (whose cells
+    exceed 64 particles once refined) in both arithmetic modes."""
     scenes = _mix()
     for s in scenes:
         if tolerance:
             s.params.flags |= abi.FLAG_TOLERANCE_MATH
     ref = None
-    for tile, blocks in ((-1, 0), (0, 0), (1, 4), (2, 16), (0, 1), (1, 0)):
+    for tile, blocks in ((-1, 0), (0, 0), (1, 4), (2, 16), (0, 1), (1, 0), (3, 0), (3, 8), (4, 0)):
         b = Batch.for_scenes(scenes, graph=True)
         ids = [b.add(s) for s in scenes]
         b.set_schedule(tile, blocks)
@@ -126,7 +126,7 @@ def test_schedule_arguments_are_checked():
     scenes = _mix()[:1]
     b = Batch.for_scenes(scenes, graph=False)
     b.add(scenes[0])
-    for tile, blocks in ((3, 0), (-2, 0), (0, 65)):
+    for tile, blocks in ((5, 0), (-2, 0), (0, 65)):
         with pytest.raises(abi.AsphError):
             b.set_schedule(tile, blocks)
     b.close()
