@@ -36,7 +36,7 @@ SPACE = {
     "query_tile": ((0, 1, 2, 3, 4), ("gather", "neighbors", "density", "force")),
     "force_blocks": ((0, 4, 8, 16), ("force",)),
 }
-# query_tile -1: the engine's own default (sub-cell tiles in adaptive batches)
+# query_tile -1: the engine's own default (the automatic query order)
 DEFAULT = {"block_stream": 256, "radix_bits": 0, "block_neighbor": 256, "query_tile": -1, "force_blocks": 0}
 
 

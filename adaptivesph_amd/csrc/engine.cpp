@@ -448,8 +448,7 @@ void Engine::invalidateGraphs()
 void Engine::buildCtx()
 {
     Ctx& c = ctx_;
-    const int tile = cfg_.launch.queryTile < 0 ? (cfg_.adaptive ? 2 : static_cast<int>(kQueryAuto))
-                                               : cfg_.launch.queryTile;
+    const int tile = cfg_.launch.queryTile < 0 ? static_cast<int>(kQueryAuto) : cfg_.launch.queryTile;
     c.qperm = tile > 0 ? qpermBuf_ : nullptr;
     c.qpos = tile > 0 ? qposBuf_ : nullptr;
     c.queryTileBits = static_cast<uint32_t>(tile);

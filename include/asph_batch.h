@@ -79,7 +79,7 @@ int asph_batch_set_launch(asph_batch* batch, uint32_t block_stream, uint32_t blo
    in x-major order (slabs of the cell); 4 per scene, the sorted order until
    the scene's waves loosen (their candidate filters keep more than 8.5
    survivors per neighbour of a particle, as the flow mixes each cell's
-   particles), then 3; -1 the default (2 for adaptive batches, 4 otherwise). force_blocks_per_cu: 0 launches the force
+   particles), then 3; -1 the default (4). force_blocks_per_cu: 0 launches the force
    kernels with one thread per particle, 1..64 as a persistent grid of that
    many blocks per compute unit. Results do not depend on the schedule. */
 int asph_batch_set_schedule(asph_batch* batch, int32_t query_tile, uint32_t force_blocks_per_cu);
