@@ -33,6 +33,22 @@ constexpr uint32_t kQueryAuto = 4;
 constexpr uint32_t kQuerySample = 64;       // one wave in 64 counts for the automatic order
 constexpr uint32_t kQueryMinHits = 65536;   // neighbours counted before a decision (~16 waves)
 
+// Wave order of the list builder: waves whose last step cost more than 1.5x
+// the sampled mean survivors, or walked lane by lane, run first (at most
+// kHeavyCap of them), so that no costly wave starts in the kernel's last
+// round and finishes alone (1M dam break over 1,000 steps, ms/step by list
+// size: none 0.997, 256 0.985, 1,024 0.960, 4,096 0.960).
+#ifndef ASPH_HEAVY_CAP
+#define ASPH_HEAVY_CAP 1024
+#endif
+#ifndef ASPH_HEAVY_FACTOR2
+#define ASPH_HEAVY_FACTOR2 3
+#endif
+constexpr uint32_t kHeavyCap = ASPH_HEAVY_CAP;
+constexpr uint32_t kHeavyFactor2 = ASPH_HEAVY_FACTOR2;  // threshold: kHeavyFactor2 / 2 x the mean cost
+constexpr uint32_t kCostPerLane = 0xFFFFFFFFU;  // wave cost of a lane-by-lane walk
+constexpr uint32_t kCostSample = 16;            // one wave in 16 feeds the mean
+
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;
 constexpr uint32_t kSceneAdaptive = 2U;

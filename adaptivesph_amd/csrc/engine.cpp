@@ -326,6 +326,11 @@ Engine::Engine(const EngineConfig& cfg)
         c.wallQueue = alloc<uint32_t>(P);
         c.wallQueueCount = alloc<uint32_t>(1);
         c.queryAny = alloc<uint32_t>(1);
+        c.wcost = alloc<uint32_t>((static_cast<size_t>(P) + 63U) / 64U + 1U);
+        c.wheavy = alloc<uint32_t>((static_cast<size_t>(P) + 63U) / 64U + 1U);
+        c.heavyList = alloc<uint32_t>(kHeavyCap);
+        c.heavyCount = alloc<uint32_t>(1);
+        c.wstat = alloc<uint32_t>(4);
         cfg_.listCap = listCapacityRounded(cfg_.listCap);
         c.listCap = cfg_.listCap;
         // wave-blocked rows: whole waves of 64 slots
@@ -381,6 +386,10 @@ Engine::Engine(const EngineConfig& cfg)
         hipCheck(hipMemset(nOrderDev_, 0, sizeof(uint32_t)), "hipMemset");
         hipCheck(hipMemset(c.nTotal, 0, sizeof(uint32_t)), "hipMemset");
         hipCheck(hipMemset(c.queryAny, 0, sizeof(uint32_t)), "hipMemset");
+        hipCheck(hipMemset(c.wcost, 0, sizeof(uint32_t) * ((static_cast<size_t>(P) + 63U) / 64U + 1U)), "hipMemset");
+        hipCheck(hipMemset(c.wheavy, 0, sizeof(uint32_t) * ((static_cast<size_t>(P) + 63U) / 64U + 1U)), "hipMemset");
+        hipCheck(hipMemset(c.heavyCount, 0, sizeof(uint32_t)), "hipMemset");
+        hipCheck(hipMemset(c.wstat, 0, sizeof(uint32_t) * 4), "hipMemset");
         if (cfg_.adaptive)
         {
             hipCheck(hipMemset(c.removal, 0, sizeof(uint32_t) * P), "hipMemset");

@@ -90,6 +90,15 @@ struct Ctx
     uint32_t* wallQueue;
     uint32_t* wallQueueCount;
     uint32_t* queryAny;  // device: some scene takes the query order this step (kPrep)
+    // list builder's wave order (heavy waves first): per wave of 64 query
+    // positions its cost in the last step (survivors, or kCostPerLane) and
+    // whether it is listed; the list; sampled cost statistics
+    // (sum, waves, threshold)
+    uint32_t* wcost;
+    uint32_t* wheavy;
+    uint32_t* heavyList;
+    uint32_t* heavyCount;
+    uint32_t* wstat;
     uint32_t* list;  // [listCap * pCap rounded up to 64], 4-entry chunks, wave-blocked: entry k of slot t
                      // at (((t / 64) * (listCap / 4) + k / 4) * 64 + t % 64) * 4 + k % 4
     uint32_t listCap;  // a multiple of 4

@@ -210,6 +210,14 @@ __global__ __launch_bounds__(256) void kPrep(Ctx c)
     if (threadIdx.x == 0)
     {
         *c.queryAny = anyQuery;
+        // heavy-wave threshold from the last step's sampled costs
+        const uint32_t sampled = c.wstat[1];
+        c.wstat[2] = sampled >= 8U ? static_cast<uint32_t>((static_cast<uint64_t>(c.wstat[0]) * kHeavyFactor2) /
+                                                            (2U * sampled))
+                                   : 0U;
+        c.wstat[0] = 0U;
+        c.wstat[1] = 0U;
+        *c.heavyCount = 0U;
         *c.nTotal = carry;
         *c.overflowCount = 0U;
         *c.wallQueueCount = 0U;
@@ -389,6 +397,25 @@ __global__ __launch_bounds__(1024) void kGather(Ctx c)
         {
             c.gbox0[t / kGroup] = F4 {lx, ly, lz, mr};
             c.gbox1[t / kGroup] = F4 {hx, hy, hz, 0.0F};
+        }
+        if ((t & 63U) == 0U && t < n)
+        {
+            // the list builder's wave order: the waves of 64 query positions
+            // that cost most in the last step are listed to run first
+            const uint32_t w = t / 64U;
+            const uint32_t cost = c.wcost[w];
+            const uint32_t thr = c.wstat[2];
+            uint32_t listed = 0U;
+            if (cost == kCostPerLane || (thr != 0U && cost > thr))
+            {
+                const uint32_t k = atomicAdd(c.heavyCount, 1U);
+                if (k < kHeavyCap)
+                {
+                    c.heavyList[k] = w;
+                    listed = 1U;
+                }
+            }
+            c.wheavy[w] = listed;
         }
     }
 }
@@ -1319,8 +1346,29 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
     F4* wstash = stash + (tid / 64U) * kTable;
     uint32_t* wslot = slots + (tid / 64U) * kTable;
     // one particle per thread (the grid covers the slots; a loop here would
-    // let the compiler hoist the pressure's constants across the walk)
-    const uint32_t tw = blockIdx.x * BS + tid;
+    // let the compiler hoist the pressure's constants across the walk). The
+    // grid's first kHeavyCap waves take the listed heavy waves (kGather),
+    // the others the waves of 64 query positions in order, skipping listed
+    // ones.
+    const uint32_t v = blockIdx.x * (BS / 64U) + tid / 64U;
+    uint32_t w;
+    if (v < kHeavyCap)
+    {
+        if (v >= min(*c.heavyCount, kHeavyCap))
+        {
+            return;
+        }
+        w = c.heavyList[v];
+    }
+    else
+    {
+        w = v - kHeavyCap;
+        if (w * 64U < nPad && c.wheavy[w] != 0U)
+        {
+            return;
+        }
+    }
+    const uint32_t tw = w * 64U + lane;
     if (tw < nPad)
     {
         const bool valid = tw < n;
@@ -1400,8 +1448,16 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
             });
         }
         const uint32_t cnt = done + (cntA - doneA) / kRow;
-        if (waveWalk && c.queryTileBits == kQueryAuto && (blockIdx.x * (BS / 64U) + tid / 64U) % kQuerySample == 0U &&
-            c.states[sw].qorder == 0U)
+        if (lane == 0U)
+        {
+            c.wcost[w] = waveWalk ? survivors : kCostPerLane;
+            if (waveWalk && w % kCostSample == 0U)
+            {
+                atomicAdd(&c.wstat[0], survivors);
+                atomicAdd(&c.wstat[1], 1U);
+            }
+        }
+        if (waveWalk && c.queryTileBits == kQueryAuto && w % kQuerySample == 0U && c.states[sw].qorder == 0U)
         {
             // the automatic query order's measure, on one wave in kQuerySample
             // (atomics of every wave on one address would serialise):
@@ -3120,15 +3176,15 @@ void launchNeighbors(const Ctx& c, const LaunchCfg& lc, hipStream_t s)
     const uint32_t b = lc.blockNeighbor;
     if (b == 64U)
     {
-        hipLaunchKernelGGL(kNeighbors<64>, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<64>, dim3(gridCover(c.nMax + 64U * kHeavyCap, b)), dim3(b), 0, s, c);
     }
     else if (b == 128U)
     {
-        hipLaunchKernelGGL(kNeighbors<128>, dim3(gridCover(c.nMax, b)), dim3(b), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<128>, dim3(gridCover(c.nMax + 64U * kHeavyCap, b)), dim3(b), 0, s, c);
     }
     else
     {
-        hipLaunchKernelGGL(kNeighbors<256>, dim3(gridCover(c.nMax, 256U)), dim3(256), 0, s, c);
+        hipLaunchKernelGGL(kNeighbors<256>, dim3(gridCover(c.nMax + 64U * kHeavyCap, 256U)), dim3(256), 0, s, c);
     }
 }
 
