@@ -308,6 +308,8 @@ def _extra_sigs():
         "asph_batch_memory": (C.c_int, [vp, P(u64), P(u64)]),
         "asph_batch_step_particles": (C.c_int, [vp, P(u32)]),
         "asph_batch_profile_step": (C.c_int, [vp, P(f32), u32, P(u32)]),
+        "asph_batch_profile_step_async": (C.c_int, [vp]),
+        "asph_batch_profile_collect": (C.c_int, [vp, P(f32), u32, P(u32), P(u32)]),
         "asph_profile_stage_name": (C.c_char_p, [u32]),
         "asph_scene_bytes": (u64, [u32, u32, u32, u32, u32]),
         "asph_grid_cells": (C.c_int, [P(Domain), f32, P(u32)]),

@@ -151,3 +151,16 @@ class Batch:
         n = C.c_uint32()
         self._check(self._lib.asph_batch_profile_step(self._h, ms, 32, C.byref(n)), "profile step")
         return {self._lib.asph_profile_stage_name(i).decode(): float(ms[i]) for i in range(n.value)}
+
+    def profile_step_async(self) -> None:
+        """An eager step with HIP events between its stages, not waited for."""
+        self._check(self._lib.asph_batch_profile_step_async(self._h), "profile step")
+
+    def profile_collect(self) -> tuple[dict[str, float], int]:
+        """Stage times (ms) averaged over the steps of profile_step_async
+        since the last collection, and their number (waits for the stream)."""
+        ms = (C.c_float * 32)()
+        n = C.c_uint32()
+        k = C.c_uint32()
+        self._check(self._lib.asph_batch_profile_collect(self._h, ms, 32, C.byref(n), C.byref(k)), "profile collect")
+        return {self._lib.asph_profile_stage_name(i).decode(): float(ms[i]) for i in range(n.value)}, int(k.value)

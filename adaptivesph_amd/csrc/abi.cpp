@@ -862,6 +862,25 @@ int asph_batch_profile_step(asph_batch* batch, float* ms, uint32_t capacity, uin
     });
 }
 
+int asph_batch_profile_step_async(asph_batch* batch)
+{
+    return guard([&] { engineOf(batch).profileStepAsync(); });
+}
+
+int asph_batch_profile_collect(asph_batch* batch, float* ms, uint32_t capacity, uint32_t* count, uint32_t* samples)
+{
+    return guard([&] {
+        need(count, "count");
+        need(samples, "samples");
+        const std::vector<float> t = engineOf(batch).collectProfiles(samples);
+        *count = static_cast<uint32_t>(t.size());
+        for (uint32_t i = 0; i < t.size() && i < capacity && ms != nullptr; ++i)
+        {
+            ms[i] = t[i];
+        }
+    });
+}
+
 const char* asph_profile_stage_name(uint32_t stage) { return asph::stageName(stage); }
 
 uint64_t asph_scene_bytes(uint32_t capacity, uint32_t boundary_count, uint32_t cells, uint32_t adaptive,

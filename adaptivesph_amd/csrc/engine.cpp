@@ -422,6 +422,13 @@ Engine::Engine(const EngineConfig& cfg)
 Engine::~Engine()
 {
     (void)hipStreamSynchronize(stream_);
+    for (auto& marks : pendingMarks_)
+    {
+        for (hipEvent_t e : marks)
+        {
+            (void)hipEventDestroy(e);
+        }
+    }
     invalidateGraphs();
     for (void* p : allocations_)
     {
@@ -1346,6 +1353,49 @@ std::vector<float> Engine::profileStep()
     for (hipEvent_t e : marks)
     {
         (void)hipEventDestroy(e);
+    }
+    return ms;
+}
+
+void Engine::profileStepAsync()
+{
+    const bool refine = anyRefineNext();
+    std::vector<hipEvent_t> marks;
+    enqueueStep(refine, &marks);
+    advanceFrames();
+    pendingMarks_.push_back(std::move(marks));
+}
+
+std::vector<float> Engine::collectProfiles(uint32_t* samples)
+{
+    synchronize();
+    std::vector<double> sum;
+    std::vector<uint32_t> n;
+    for (const auto& marks : pendingMarks_)
+    {
+        for (size_t k = 0; k + 1 < marks.size(); ++k)
+        {
+            float t = 0.F;
+            hipCheck(hipEventElapsedTime(&t, marks[k], marks[k + 1]), "hipEventElapsedTime");
+            if (sum.size() <= k)
+            {
+                sum.resize(k + 1, 0.0);
+                n.resize(k + 1, 0U);
+            }
+            sum[k] += t;
+            n[k] += 1U;
+        }
+        for (hipEvent_t e : marks)
+        {
+            (void)hipEventDestroy(e);
+        }
+    }
+    *samples = static_cast<uint32_t>(pendingMarks_.size());
+    pendingMarks_.clear();
+    std::vector<float> ms(sum.size());
+    for (size_t k = 0; k < sum.size(); ++k)
+    {
+        ms[k] = static_cast<float>(sum[k] / n[k]);
     }
     return ms;
 }

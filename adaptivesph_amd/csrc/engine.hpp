@@ -160,6 +160,11 @@ public:
     void setLaunch(const LaunchCfg& lc);
     // One eager step with an event between stages (stage times in ms).
     std::vector<float> profileStep();
+    // The same step left in flight (no wait); collectProfiles() waits for the
+    // stream and returns each stage's time averaged over the steps so
+    // enqueued since the last collection (their number in *samples).
+    void profileStepAsync();
+    std::vector<float> collectProfiles(uint32_t* samples);
     // One step bracketed by events (ms).
     float timedStep();
     void setUseGraph(bool on);
@@ -226,6 +231,7 @@ private:
     uint32_t* infoU_ = nullptr;
     uint32_t* diagTotal_ = nullptr;
     bool debugCapture_ = false;
+    std::vector<std::vector<hipEvent_t>> pendingMarks_;  // profileStepAsync's events
     hipGraphExec_t graphPlain_ = nullptr;
     hipGraphExec_t graphRefine_ = nullptr;
     hipGraph_t graphPlainG_ = nullptr;

@@ -151,14 +151,15 @@ def cpu_baseline(scene_obj, steps: int, runs: int) -> dict:
 def timed_steps(batch, steps: int, samples: int, count_particles: bool) -> tuple[float, int, dict]:
     """The timed region: `steps` steps, of which `samples` (spread evenly over
     the region) run eagerly with HIP events recorded on the engine's stream
-    between their stages, the others as graph replays. Returns the elapsed
-    wall time (device synchronised on both sides), the particles updated (each
-    step's count when count_particles, else 0) and each stage's device time
-    averaged over the sampled steps. The per-stage times therefore cover the
-    same steps as the throughput, as a step's cost drifts with the flow."""
+    between their stages, the others as graph replays; nothing waits for the
+    device inside the region (the events are read after it). Returns the
+    elapsed wall time (device synchronised on both sides), the particles
+    updated (each step's count when count_particles, else 0) and each stage's
+    device time averaged over the sampled steps. The per-stage times therefore
+    cover the same steps as the throughput, as a step's cost drifts with the
+    flow."""
     samples = max(1, min(samples, steps))
     every = steps // samples  # step k is sampled when k % every == every - 1
-    stages: dict = {}
     taken = 0
     particles = 0
     batch.synchronize()
@@ -166,8 +167,7 @@ def timed_steps(batch, steps: int, samples: int, count_particles: bool) -> tuple
     k = 0
     while k < steps:
         if k % every == every - 1 and taken < samples:
-            for name, v in batch.profile_step().items():
-                stages.setdefault(name, []).append(v)
+            batch.profile_step_async()
             taken += 1
             run = 1
         else:
@@ -179,7 +179,8 @@ def timed_steps(batch, steps: int, samples: int, count_particles: bool) -> tuple
         k += run
     batch.synchronize()
     elapsed = time.perf_counter() - t0
-    return elapsed, particles, {name: sum(v) / len(v) for name, v in stages.items()}
+    stages, _ = batch.profile_collect()
+    return elapsed, particles, stages
 
 
 def counters_key(workload: str, math: str) -> str:

@@ -94,6 +94,12 @@ int asph_batch_memory(asph_batch* batch, uint64_t* allocated, uint64_t* particle
 /* One eager step with a HIP event between stages; ms[i] receives the time of
    stage i (up to capacity entries), *count the number of stages. */
 int asph_batch_profile_step(asph_batch* batch, float* ms, uint32_t capacity, uint32_t* count);
+/* The same eager step, enqueued without waiting (between graph-replayed steps
+   of a timed run); asph_batch_profile_collect waits for the batch's stream
+   and returns each stage's time averaged over the steps so enqueued since the
+   last collection (*samples of them). */
+int asph_batch_profile_step_async(asph_batch* batch);
+int asph_batch_profile_collect(asph_batch* batch, float* ms, uint32_t capacity, uint32_t* count, uint32_t* samples);
 const char* asph_profile_stage_name(uint32_t stage);
 
 /* Device bytes one scene of this shape occupies inside a batch
