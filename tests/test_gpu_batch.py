@@ -60,6 +60,29 @@ def test_graph_replays_match_eager_every_step():
             assert_bitexact(g.fluid(a), e.fluid(b), f"{s.name} replay {step}")
 
 
+def test_profiled_steps_between_replays_keep_results():
+    """Eager steps with stage events enqueued between graph replays without
+    waiting (the benchmark's timed region) advance the scenes exactly as
+    plain steps do; the collected stage times cover every such step."""
+    scenes = _mix()
+    g = Batch.for_scenes(scenes, graph=True)
+    p = Batch.for_scenes(scenes, graph=True)
+    gi = [g.add(s) for s in scenes]
+    pi = [p.add(s) for s in scenes]
+    g.step(9)
+    for _ in range(3):
+        p.step(2)
+        p.profile_step_async()
+    stages, samples = p.profile_collect()
+    assert samples == 3
+    assert {"neighbors", "force"} <= set(stages) and all(v >= 0.0 for v in stages.values())
+    for s, a, b in zip(scenes, gi, pi):
+        assert_bitexact(g.fluid(a), p.fluid(b), f"{s.name} after profiled steps")
+    assert p.profile_collect()[1] == 0
+    g.close()
+    p.close()
+
+
 def test_continuous_admission_and_removal():
     scenes = _mix()
     batch = Batch.for_scenes(scenes, graph=True, slack=1.5)
