@@ -228,6 +228,16 @@ def stage_figures(stage_ms: dict, particles: int, pairs: float | None, workload:
     return out
 
 
+def binding_resource(stage: str, valu_frac: float | None) -> str:
+    """What limits the stage's main kernel, read from its counters."""
+    if (valu_frac or 0.0) < 0.8:
+        return "vector memory path (per-lane 16-byte gathers through L1; see profiles/valu_*.md)"
+    if stage == "density":
+        return ("instruction issue: the wave filter's VALU, LDS, scalar and texture-path work together "
+                "(VALU issue share near 1; see profiles/valu_*.md)")
+    return "VALU issue (instruction-bound pair loop; see valu_issue_frac)"
+
+
 def roofline_of(stages: dict) -> dict:
     """The bench line's roofline object for the dominant kernel: its
     algorithmic bytes against HBM (the contract's figure), beside the binding
@@ -246,9 +256,7 @@ def roofline_of(stages: dict) -> dict:
         "traffic": c.get("dram_bytes_per_launch"),
         "algorithmic_bytes_per_launch": f["algorithmic_bytes"],
         "launch_ms": f["launch_ms"],
-        "binding_resource": ("VALU issue (instruction-bound neighbour loop; see valu_issue_frac)"
-                             if (c.get("valu_issue_frac") or 0.0) >= 0.8 else
-                             "vector memory path (per-lane 16-byte gathers through L1; see profiles/valu_*.md)"),
+        "binding_resource": binding_resource(dom, c.get("valu_issue_frac")),
         "valu_issue_frac": c.get("valu_issue_frac"),
         "fp32_tflops": f.get("tflops"),
         "fp32_peak_tflops": FP32_PEAK_TFLOPS,
