@@ -49,6 +49,31 @@ constexpr uint32_t kHeavyFactor2 = ASPH_HEAVY_FACTOR2;  // threshold: kHeavyFact
 constexpr uint32_t kCostPerLane = 0xFFFFFFFFU;  // wave cost of a lane-by-lane walk
 constexpr uint32_t kCostSample = 16;            // one wave in 16 feeds the mean
 
+// Neighbour-list row width: each slot's entries are stored kListRowEntries
+// at a time (ASPH_LIST_ROW_CHUNKS 16-byte chunks side by side); list
+// capacities are rounded up to a multiple of it. Two chunks per row fill
+// whole 32-byte sectors at each write (1M dam break: list-builder writes
+// 628 -> 492 MB per step), but each 16-byte list load then spans twice the
+// cache lines and the tolerance-mode force kernel ran 6 % slower, so one.
+#ifndef ASPH_LIST_ROW_CHUNKS
+#define ASPH_LIST_ROW_CHUNKS 1
+#endif
+constexpr uint32_t kListRowChunks = ASPH_LIST_ROW_CHUNKS;
+constexpr uint32_t kListRowEntries = 4U * kListRowChunks;
+
+// kNeighbors deals its regular waves to the XCDs (each with its own L2) in
+// runs of kNeighborRunWaves consecutive query waves, so that the waves that
+// read the same candidate cells share an L2 while costly regions still
+// spread over every XCD (0: the dispatch order, workgroup i on XCD i mod 8).
+// 1M dam break, ms/step over 200 steps by run length in workgroups of 4
+// waves: none 0.905, 16: 0.904, 64: 0.895, 256: 0.884, one contiguous eighth
+// per XCD 0.891 (the 256-scene batch 4 % slower with eighths); list-builder
+// reads (FETCH_SIZE) 770 -> 648 MB at 256.
+#ifndef ASPH_NEIGHBOR_RUN_WAVES
+#define ASPH_NEIGHBOR_RUN_WAVES 1024
+#endif
+constexpr uint32_t kNeighborRunWaves = ASPH_NEIGHBOR_RUN_WAVES;
+
 // Scene flags (SceneDev::flags)
 constexpr uint32_t kSceneActive = 1U;
 constexpr uint32_t kSceneAdaptive = 2U;

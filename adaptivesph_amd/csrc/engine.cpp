@@ -230,8 +230,8 @@ void validateParams(const asph_parameters& p)
 
 uint32_t listCapacityRounded(uint32_t listCap)
 {
-    const uint32_t k = listCap == 0U ? 4U : listCap;
-    return ((k + 3U) / 4U) * 4U;
+    const uint32_t k = listCap == 0U ? kListRowEntries : listCap;
+    return ((k + kListRowEntries - 1U) / kListRowEntries) * kListRowEntries;
 }
 
 uint64_t sceneBytes(uint32_t capacity, uint32_t boundaryCount, uint32_t cells, bool adaptive, uint32_t listCap)

@@ -729,25 +729,35 @@ typedef const __attribute__((address_space(4))) uint32_t ConstU32;
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup i runs on
 // XCD i mod 8), and each XCD has its own L2. The neighbour-list consumers
-// renumber their workgroups so that each XCD takes one contiguous eighth of
-// the n sorted particles: neighbouring particles (which gather the same
-// neighbour data) then share an L2. The eighths are taken of the workgroups
-// that have particles (the grid is sized for the pool's capacity, which can
-// be many times n). Results do not depend on the numbering. (kNeighbors
-// reads its candidates through the scalar cache and measured faster without
-// the renumbering.)
+// renumber their workgroups so that each XCD takes runs of kForceRunWaves
+// consecutive waves of the sorted particles, the runs dealt to the XCDs in
+// turn: neighbouring particles (which gather the same neighbour data) then
+// share an L2, while a costly region (refined cells) still spreads over every
+// XCD. The runs are taken of the workgroups that have particles (the grid is
+// sized for the pool's capacity, which can be many times n). Results do not
+// depend on the numbering. Force stage by run length in waves (ms; 500k
+// adaptive tank / 256-scene batch / 1M tolerance mode): dispatch order
+// 0.58 / 8.70 / 0.248, 32: 0.54 / 8.28 / 0.246, 64: 0.54 / 8.33 / 0.245,
+// 128: 0.55 / 8.42 / 0.242, 256: 0.65 / 8.67 / 0.243, one contiguous eighth
+// per XCD 0.65 / 9.98 / 0.248. (kNeighbors deals its waves in longer runs:
+// kNeighborRunWaves.)
+#ifndef ASPH_FORCE_RUN_WAVES
+#define ASPH_FORCE_RUN_WAVES 64
+#endif
+constexpr uint32_t kForceRunWaves = ASPH_FORCE_RUN_WAVES;
 __device__ __forceinline__ uint32_t xcdBlock(uint32_t n)
 {
     constexpr uint32_t kXcds = 8;
     const uint32_t needed = (n + blockDim.x - 1U) / blockDim.x;
     const uint32_t used = needed < gridDim.x ? needed : gridDim.x;
-    const uint32_t per = used / kXcds;
     const uint32_t b = blockIdx.x;
-    if (b >= per * kXcds)
+    const uint32_t G = kForceRunWaves / (blockDim.x / 64U) > 0U ? kForceRunWaves / (blockDim.x / 64U) : 1U;
+    if (b >= (used / (kXcds * G)) * (kXcds * G))
     {
-        return b;
+        return b;  // the tail past the last whole round of runs keeps its order
     }
-    return (b % kXcds) * per + b / kXcds;
+    const uint32_t k = b / kXcds;
+    return (k / G) * (kXcds * G) + (b % kXcds) * G + k % G;
 }
 
 // Neighbour-list entries whose data are loaded together before the pair math
@@ -782,11 +792,9 @@ constexpr uint32_t kForceUniformBatch = ASPH_FORCE_UNIFORM_BATCH;  // a multiple
 // neighbour list; phase 2 accumulates the density over that list in the same
 // order with every lane active.
 //
-// List layout: chunks of kChunk = 4 entries, chunk-major then particle:
-// entry k of sorted slot t is list[((k / 4) * pCap + t) * 4 + k % 4]. A
-// particle's chunk is one 16-byte segment, written whole from an LDS staging
-// ring; at a given k the wave's lanes read 64 adjacent segments (1 KiB,
-// coalesced), four entries per 16-byte load.
+// List entries travel in chunks of kChunk = 4 (one 16-byte segment, written
+// whole from an LDS staging ring, read with one 16-byte load); the layout of
+// the chunks is listChunk's.
 constexpr uint32_t kChunk = 4;
 
 // The consumers stream the list through once each; non-temporal loads keep it
@@ -806,13 +814,19 @@ __device__ __forceinline__ void storeChunk(uint4* p, uint32_t a, uint32_t b, uin
 }
 
 // Wave-blocked list layout: the 64 slots of an aligned wave (t & ~63) own
-// listCap * 64 consecutive entries, chunk row by chunk row; chunk k / 4 of
-// slot t is the 16 bytes at chunk index ((t / 64) * (listCap / 4) + k / 4) * 64
-// + t % 64. A wave reads or writes 1 KiB contiguously per chunk row, and its
-// whole list is one contiguous block (few pages per wave, whatever the pool).
+// listCap * 64 consecutive entries, row by row; a row holds kRowEntries
+// entries of each of the 64 slots (kRowChunks 16-byte chunks per slot, side
+// by side), so chunk k / 4 of slot t is the 16 bytes at chunk index
+// ((t / 64) * (listCap / kRowEntries) + k / kRowEntries) * 64 * kRowChunks
+// + (t % 64) * kRowChunks + (k / 4) % kRowChunks. A wave reads or writes its
+// rows contiguously, and its whole list is one contiguous block (few pages
+// per wave, whatever the pool).
+constexpr uint32_t kRowChunks = kListRowChunks;
+constexpr uint32_t kRowEntries = kListRowEntries;
 __device__ __forceinline__ size_t listChunk(const Ctx& c, uint32_t t, uint32_t k)
 {
-    return (static_cast<size_t>(t >> 6) * (c.listCap / kChunk) + k / kChunk) * 64U + (t & 63U);
+    return (static_cast<size_t>(t >> 6) * (c.listCap / kRowEntries) + k / kRowEntries) * (64U * kRowChunks) +
+           (t & 63U) * kRowChunks + (k / kChunk) % kRowChunks;
 }
 
 __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint32_t k)
@@ -821,10 +835,10 @@ __device__ __forceinline__ const uint32_t* listAt(const Ctx& c, uint32_t t, uint
 }
 
 // Chunk k / 4 of a list whose first chunk is `row0` (listChunk(c, t, 0)):
-// chunk rows lie 64 chunks apart, so the per-chunk address is one add.
+// rows lie 64 * kRowChunks chunks apart.
 __device__ __forceinline__ const uint32_t* listRowAt(const uint4* row0, uint32_t k)
 {
-    return reinterpret_cast<const uint32_t*>(row0 + (k / kChunk) * 64U);
+    return reinterpret_cast<const uint32_t*>(row0 + (k / kRowEntries) * (64U * kRowChunks) + (k / kChunk) % kRowChunks);
 }
 
 // The T at the start of 16-byte record j of a particle array, addressed by a
@@ -1333,7 +1347,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
     // staging ring of kRing entries per thread, entry-major ([e][thread]) so
     // that the lanes' writes fall in distinct banks; per wave, the survivor
     // table of the candidate filter
-    constexpr uint32_t kRing = 2U * kChunk;
+    constexpr uint32_t kRing = 2U * kRowEntries;
     constexpr uint32_t kTable = 64U + kStashPad;
     __shared__ uint32_t stage[kRing * BS];
     __shared__ F4 stash[(BS / 64U) * kTable];
@@ -1350,7 +1364,29 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
     // grid's first kHeavyCap waves take the listed heavy waves (kGather),
     // the others the waves of 64 query positions in order, skipping listed
     // ones.
-    const uint32_t v = blockIdx.x * (BS / 64U) + tid / 64U;
+    uint32_t blk = blockIdx.x;
+    if constexpr (kNeighborRunWaves != 0U)
+    {
+        // regular workgroups (past the heavy ones; kHeavyCap / (BS / 64) is a
+        // multiple of 8, so a workgroup keeps its XCD) in runs of G
+        // consecutive workgroups per XCD, the runs dealt to the XCDs in turn;
+        // the tail past the last whole round of runs keeps its order
+        constexpr uint32_t kHeavyBlocks = kHeavyCap / (BS / 64U);
+        constexpr uint32_t G = kNeighborRunWaves / (BS / 64U) > 0U ? kNeighborRunWaves / (BS / 64U) : 1U;
+        if (blk >= kHeavyBlocks)
+        {
+            const uint32_t rb = blk - kHeavyBlocks;
+            const uint32_t needed = ((n + 63U) / 64U + BS / 64U - 1U) / (BS / 64U);
+            const uint32_t avail = gridDim.x - kHeavyBlocks;
+            const uint32_t used = needed < avail ? needed : avail;
+            if (rb < (used / (8U * G)) * (8U * G))
+            {
+                const uint32_t k = rb / 8U;
+                blk = kHeavyBlocks + (k / G) * (8U * G) + (rb % 8U) * G + k % G;
+            }
+        }
+    }
+    const uint32_t v = blk * (BS / 64U) + tid / 64U;
     uint32_t w;
     if (v < kHeavyCap)
     {
@@ -1398,28 +1434,35 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         uint32_t done = 0;
         char* ring = reinterpret_cast<char*>(stage);
         auto staged = [&](uint32_t off) { return *reinterpret_cast<const uint32_t*>(ring + off); };
-        // the thread's next chunk (listChunk): chunk rows of a wave's block
-        // are 64 chunks apart
+        // the thread's next row (listChunk): rows of a wave's block are
+        // 64 * kRowChunks chunks apart
         uint4* cptr = reinterpret_cast<uint4*>(c.list) + listChunk(c, pq, 0U);
-        constexpr uint32_t cstep = 64U;
+        constexpr uint32_t cstep = 64U * kRowChunks;
         auto push = [&](bool hit, uint32_t j) {
             *reinterpret_cast<uint32_t*>(ring + (cntA & kRingMask)) = j;
             cntA += hit ? kRow : 0U;
         };
-        // writes the oldest staged chunk once it is full (called after at most
-        // kChunk pushes, so the ring never holds more than kRing - 1 entries)
+        // writes the oldest staged row once it is full (called after at most
+        // kChunk pushes, so the ring never holds more than kRing - 1 entries);
+        // the row's chunks leave together, so that the sectors they fill are
+        // written whole
         auto drain = [&]() {
-            if (cntA - doneA >= kChunk * kRow)
+            if (cntA - doneA >= kRowEntries * kRow)
             {
                 if (done < K)
                 {
-                    // done is a multiple of kChunk: the chunk does not wrap
-                    const uint32_t e0 = doneA & kRingMask;
-                    storeChunk(cptr, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow), staged(e0 + 3U * kRow));
+                    // done is a multiple of kRowEntries: the row does not wrap
+#pragma unroll
+                    for (uint32_t h = 0; h < kRowChunks; ++h)
+                    {
+                        const uint32_t e0 = (doneA + h * kChunk * kRow) & kRingMask;
+                        storeChunk(cptr + h, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow),
+                                   staged(e0 + 3U * kRow));
+                    }
                     cptr += cstep;
                 }
-                done += kChunk;
-                doneA += kChunk * kRow;
+                done += kRowEntries;
+                doneA += kRowEntries * kRow;
             }
         };
         // A wave whose queries lie far apart (spray, a sparse surface) would
@@ -1475,11 +1518,19 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kN
         }
         if (valid)
         {
-            if (cnt <= K && cnt > done)
+            if (cnt <= K)
             {
-                // partial last chunk (entries past cnt are never read)
-                const uint32_t e0 = doneA & kRingMask;
-                storeChunk(cptr, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow), staged(e0 + 3U * kRow));
+                // partial last row (entries past cnt are never read)
+#pragma unroll
+                for (uint32_t h = 0; h < kRowChunks; ++h)
+                {
+                    if (cnt > done + h * kChunk)
+                    {
+                        const uint32_t e0 = (doneA + h * kChunk * kRow) & kRingMask;
+                        storeChunk(cptr + h, staged(e0), staged(e0 + kRow), staged(e0 + 2U * kRow),
+                                   staged(e0 + 3U * kRow));
+                    }
+                }
             }
             c.ncount[t] = cnt;
             if (cnt > K)
