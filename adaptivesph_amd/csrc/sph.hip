@@ -882,10 +882,16 @@ __device__ __forceinline__ float waveMaxF(float v)
 // neighbour sequence. push(hit, j) is called for every survivor (hit: the
 // lane accepts slot j), drain() after every four.
 //
+// The exact test needs no per-survivor conditions besides d^2 <= (2 h_j)^2:
+// the table entries past the S survivors of a batch (read by its last round
+// of four) carry a support of -1, which no distance passes, and a lane whose
+// own box misses the cell tests with a NaN x coordinate, whose d^2 is NaN and
+// fails the comparison.
+//
 // The box distance is a lower bound of every lane's distance in FP32 as well:
 // each rounded difference, square and sum is monotonic in its operands and
 // uses the same association as the exact test; the margin is extra.
-constexpr uint32_t kStashPad = 4;  // table entries past 64 read (and ignored) by the last round
+constexpr uint32_t kStashPad = 4;  // table entries past 64: sentinels read by the last round
 
 template <class P, class D>
 __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& sc, uint32_t lost, bool valid,
@@ -923,6 +929,7 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
                 const uint32_t cell = base + static_cast<uint32_t>(x + y * gx + z * gxgy);
                 const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
                 const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+                const float qxc = inbox ? q.x : __builtin_nanf("");
                 for (uint32_t j0 = lo; j0 < hi; j0 += 64U)
                 {
                     const uint32_t j = j0 + lane;
@@ -948,6 +955,10 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
                         wstash[r] = o;
                         wslot[r] = j;
                     }
+                    if (lane < 4U)
+                    {
+                        wstash[S + lane].w = -1.0F;  // sentinels of the last round
+                    }
                     for (uint32_t r = 0; r < S; r += 4U)
                     {
                         F4 oc[4];
@@ -961,12 +972,12 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
 #pragma unroll
                         for (uint32_t u = 0; u < 4U; ++u)
                         {
-                            const float dx = oc[u].x - q.x;
+                            const float dx = oc[u].x - qxc;
                             const float dy = oc[u].y - q.y;
                             const float dz = oc[u].z - q.z;
                             const float d2 = dot3w(dx, dy, dz, dx, dy, dz);
-                            // bitwise: every lane tests, no branch
-                            push((d2 <= oc[u].w) & inbox & (r + u < S), jj[u]);
+                            // every lane tests, no branch
+                            push(d2 <= oc[u].w, jj[u]);
                         }
                         drain();
                     }
