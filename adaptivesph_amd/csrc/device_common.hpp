@@ -72,6 +72,13 @@ constexpr uint32_t kListRowEntries = 4U * kListRowChunks;
 #ifndef ASPH_NEIGHBOR_RUN_WAVES
 #define ASPH_NEIGHBOR_RUN_WAVES 1024
 #endif
+
+// Waves per SIMD that kNeighbors' register budget is fitted to (80 VGPRs at
+// 6). 1M dam break, ms/step over 200 steps: 6: 0.883, 7 (72 VGPRs, 32 B of
+// spills): 0.891, 8 (64 VGPRs, 100 B of spills): 0.912.
+#ifndef ASPH_NEIGHBOR_WAVES
+#define ASPH_NEIGHBOR_WAVES 6
+#endif
 constexpr uint32_t kNeighborRunWaves = ASPH_NEIGHBOR_RUN_WAVES;
 
 // Scene flags (SceneDev::flags)

@@ -1353,7 +1353,7 @@ __device__ __forceinline__ float listDensityFast(const Ctx& c, const SceneDev& s
 constexpr uint32_t kMaxUnionCells = ASPH_MAX_UNION_CELLS;
 
 template <uint32_t BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(6))) void kNeighbors(Ctx c)
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ASPH_NEIGHBOR_WAVES))) void kNeighbors(Ctx c)
 {
     // staging ring of kRing entries per thread, entry-major ([e][thread]) so
     // that the lanes' writes fall in distinct banks; per wave, the survivor
