@@ -898,7 +898,6 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
                                                  const CellBox& b, const F4& q, F4* wstash, uint32_t* wslot,
                                                  uint32_t lane, P&& push, D&& drain, uint32_t& survivors)
 {
-    ConstU32* off = (ConstU32*)(c.cellOff);
     const int ux0 = waveMinI(valid ? b.x0 : 0x7FFFFFFF);
     const int uy0 = waveMinI(valid ? b.y0 : 0x7FFFFFFF);
     const int uz0 = waveMinI(valid ? b.z0 : 0x7FFFFFFF);
@@ -914,21 +913,52 @@ __device__ __forceinline__ void waveWalkFiltered(const Ctx& c, const SceneDev& s
     const int gx = sc.gx;
     const int gxgy = sc.gx * sc.gy;
     const uint32_t base = sc.cellBase;
-    for (int x = ux0; x <= ux1; ++x)
+    // the union box's cells in visiting order (z fastest, then y, then x),
+    // 64 at a time: lane l loads the slot range of the group's cell l in one
+    // vector load, and the walk takes each cell's range from its lane, so no
+    // cell waits on a load of its own range
+    const uint32_t nz = static_cast<uint32_t>(uz1 - uz0 + 1);
+    const uint32_t nyz = static_cast<uint32_t>(uy1 - uy0 + 1) * nz;
+    const uint32_t ncell = static_cast<uint32_t>(ux1 - ux0 + 1) * nyz;
+    int x = ux0;
+    int y = uy0;
+    int z = uz0;
+    for (uint32_t k0 = 0; k0 < ncell; k0 += 64U)
     {
-        for (int y = uy0; y <= uy1; ++y)
+        uint32_t vlo = 0;
+        uint32_t vhi = 0;
+        const uint32_t kl = k0 + lane;
+        if (kl < ncell)
         {
-            for (int z = uz0; z <= uz1; ++z)
+            const uint32_t lx = kl / nyz;
+            const uint32_t ly = (kl - lx * nyz) / nz;
+            const uint32_t lz = kl - lx * nyz - ly * nz;
+            const uint32_t cl = base + static_cast<uint32_t>((ux0 + static_cast<int>(lx)) +
+                                                             (uy0 + static_cast<int>(ly)) * gx +
+                                                             (uz0 + static_cast<int>(lz)) * gxgy);
+            vlo = max(c.cellOff[cl], lost);
+            vhi = c.cellOff[cl + 1U];
+        }
+        const uint32_t kn = min(ncell - k0, 64U);
+        for (uint32_t k = 0; k < kn; ++k)
+        {
+            const bool inbox = valid && x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1;
+            if (++z > uz1)
             {
-                const bool inbox =
-                    valid && x >= b.x0 && x <= b.x1 && y >= b.y0 && y <= b.y1 && z >= b.z0 && z <= b.z1;
-                if (__ballot(inbox ? 1 : 0) == 0ULL)
+                z = uz0;
+                if (++y > uy1)
                 {
-                    continue;
+                    y = uy0;
+                    ++x;
                 }
-                const uint32_t cell = base + static_cast<uint32_t>(x + y * gx + z * gxgy);
-                const uint32_t lo = __builtin_amdgcn_readfirstlane(max(off[cell], lost));
-                const uint32_t hi = __builtin_amdgcn_readfirstlane(off[cell + 1U]);
+            }
+            if (__ballot(inbox ? 1 : 0) == 0ULL)
+            {
+                continue;
+            }
+            {
+                const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vlo), static_cast<int>(k)));
+                const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(vhi), static_cast<int>(k)));
                 const float qxc = inbox ? q.x : __builtin_nanf("");
                 for (uint32_t j0 = lo; j0 < hi; j0 += 64U)
                 {
