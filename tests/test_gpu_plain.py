@@ -324,3 +324,46 @@ def test_morris_laminar_viscosity_bitexact():
     for _ in range(s.steps):
         plain.update(s.dt)
     assert not np.array_equal(plain.fluid()["velocities"], sim.fluid()["velocities"])
+
+
+def _union_cells(keys: np.ndarray, dims: tuple[int, int, int]) -> np.ndarray:
+    """Cells in the union of the +-1 cell boxes of each run of 64 consecutive
+    sorted slots (the list builder's waves while the scene keeps the sorted
+    query order), clamped to the grid."""
+    gx, gy, gz = dims
+    k = keys.astype(np.int64)
+    x, y, z = k % gx, (k // gx) % gy, k // (gx * gy)
+    out = []
+    for w in range(0, len(k), 64):
+        sl = slice(w, w + 64)
+        n = 1
+        for c, g in ((x[sl], gx), (y[sl], gy), (z[sl], gz)):
+            n *= min(int(c.max()) + 1, g - 1) - max(int(c.min()) - 1, 0) + 1
+        out.append(n)
+    return np.array(out)
+
+
+def test_sparse_scene_waves_over_many_cells():
+    """A sparse cloud (about one particle per two cells): 64 consecutive
+    sorted particles span several rows of cells, so the list builder's waves
+    walk union boxes of more than 64 cells, whose slot ranges arrive in
+    groups of 64 cells, one per lane (and the widest waves walk lane by lane).
+    Bit-exact against the oracle over several steps."""
+    s = scene.dam_break((16, 16, 16), 0.01, (0.48, 0.48, 0.48), seed=31, steps=5, name="sparse_cloud")
+    rng = np.random.default_rng(31)
+    st = {"positions": np.zeros((s.count, 4), np.float32),
+          "velocities": np.zeros((s.count, 4), np.float32)}
+    st["positions"][:, :3] = rng.uniform(0.04, 0.44, (s.count, 3))
+    st["velocities"][:, :3] = rng.uniform(-0.05, 0.05, (s.count, 3))
+    sim, ref = _pair(s)
+    sim.set_fluid(dict(st))
+    ref.set_fluid(dict(st))
+    for i in range(s.steps):
+        sim.update(s.dt)
+        ref.update(s.dt)
+        if i == 0:
+            cells = _union_cells(ref.grid()["keys"], ref.grid_dims())
+            wide = int(np.count_nonzero((cells > 64) & (cells <= 384)))
+            print(f"waves {len(cells)}, 65-384 union cells: {wide}, over 384: {int(np.count_nonzero(cells > 384))}")
+            assert wide >= 8, cells
+    assert_bitexact(sim.fluid(), ref.fluid(), "sparse cloud")
