@@ -313,11 +313,37 @@ def run_single(args) -> dict:
     out["roofline"] = roofline_of(stages)
     out["kernels"] = stages
     out["stage_ms"] = {k: round(v, 4) for k, v in stage_ms.items()}
+    if headline and args.math == "exact" and not args.no_tolerance_leg:
+        # the same workload in the tolerance mode (hardware reciprocals in the
+        # density and force sums; within 1e-5 of the FP32 oracle), beside the
+        # bit-exact value
+        out["tolerance_mode"] = measure_tolerance(args)
     if headline and not args.no_batch:
         # the second half of the headline metric: 256 scenes stepped together
         out["batched"] = measure_batch(args, with_sequential=True)
     out["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(s, args.cpu_steps, args.cpu_runs)
     return out
+
+
+def measure_tolerance(args) -> dict:
+    """The default workload stepped the same way in the tolerance mode."""
+    from adaptivesph_amd.batch import Batch
+
+    scenes, workload = make_workload(args.workload, args.merged_h, "tolerance")
+    s = scenes[0]
+    batch = Batch.for_scenes(scenes, graph=not args.no_graph, list_capacity=args.list_capacity)
+    batch.add(s)
+    batch.step(args.warmup)
+    batch.synchronize()
+    elapsed, particles, stage_ms = timed_steps(batch, args.steps, args.profile_steps, s.adaptive)
+    batch.close()
+    particles = s.count * args.steps
+    log(f"tolerance mode: {elapsed / args.steps * 1e3:.4f} ms/step")
+    return {"math": "tolerance", "value": particles / elapsed, "unit": "particle-updates/s",
+            "ms_per_step": elapsed / args.steps * 1e3, "steps": args.steps, "workload": workload,
+            "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+            "accuracy": "density, pressure and acceleration within 1e-5 of the FP32 oracle; keys, order, "
+                        "neighbour sets and refinement decisions bit-exact (tests/test_gpu_tolerance.py)"}
 
 
 def measure_batch(args, with_sequential: bool) -> dict:
@@ -525,6 +551,8 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=50, help="CPU baseline: steps per run")
     ap.add_argument("--cpu-runs", type=int, default=3, help="CPU baseline: runs (the median is reported)")
     ap.add_argument("--no-batch", action="store_true", help="default workload: leave out the batched leg")
+    ap.add_argument("--no-tolerance-leg", action="store_true",
+                    help="default workload: leave out the tolerance-mode measurement beside the exact one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-sequential", action="store_true",
