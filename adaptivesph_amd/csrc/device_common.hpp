@@ -281,13 +281,11 @@ __device__ __forceinline__ bool sqrtInRange(float x) { return x == 0.0F || (x >=
 // below are these with q = d / h
 __device__ __forceinline__ float wendlandQ(float q, float coefOverH3)
 {
-    if (q > 2.0F)
-    {
-        return 0.0F;
-    }
+    // evaluated for every q and selected (no branch around the terms)
     const float tmp = 1.0F - 0.5F * q;
     const float tmp4 = (tmp * tmp) * (tmp * tmp);
-    return (coefOverH3 * tmp4) * (2.0F * q + 1.0F);
+    const float w = (coefOverH3 * tmp4) * (2.0F * q + 1.0F);
+    return q > 2.0F ? 0.0F : w;
 }
 
 __device__ __forceinline__ float wendlandGradQ(float q, float dcoefOverH4)

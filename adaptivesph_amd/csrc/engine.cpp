@@ -161,13 +161,16 @@ namespace
 inline float hmin(float a, float b) { return (b < a) ? b : a; }
 inline float hmax(float a, float b) { return (a < b) ? b : a; }
 
-// |v| in [2^-40, 2^40] (kDivLo, kDivHi of device_common.hpp): a divisor the
-// device may divide by through its reciprocal with the quotient's exact bits
+// |v| in [2^-40, 2^39]: a divisor the device may divide by through its
+// reciprocal with the quotient's exact bits (kDivLo, kDivHi of
+// device_common.hpp bound that at [2^-40, 2^40]), and a smoothing length whose
+// support (2 h)^2 stays within 2^80, so that the uniform density loop's
+// distances lie in the domain of its square root and quotient
 inline bool sharedDivisor(float v)
 {
     uint32_t u;
     std::memcpy(&u, &v, sizeof(u));
-    return ((u & 0x7FFFFFFFU) - 0x2B800000U) <= 0x28000000U;
+    return ((u & 0x7FFFFFFFU) - 0x2B800000U) <= 0x27800000U;
 }
 
 // calculateCellIndex + flattenCellIndex (NeighborGrid.cuh:166-177) on the host.
@@ -634,7 +637,7 @@ uint32_t Engine::addScene(const asph_parameters& p, const asph_vec4* positions, 
               ((p.flags & ASPH_FLAG_TOLERANCE_MATH) != 0U ? kSceneToleranceMath : 0U) |
               // refinement is the only writer of h and m; the uniform loops
               // divide by h0 through its reciprocal, valid for h0 in
-              // [2^-40, 2^40]
+              // [2^-40, 2^39]
               (adaptive || !sharedDivisor(p.base_smoothing_radius) ? 0U : kSceneUniformHM);
     d.tpb = p.threads_per_block;
     if (r != nullptr)

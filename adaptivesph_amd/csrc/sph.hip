@@ -1142,13 +1142,16 @@ __device__ __forceinline__ void finishDensity(const Ctx& c, const SceneDev& sc, 
 }
 
 // sqrtf(d2) / h0 through sqrtRanged and y0 = divRecip(h0), h0 in
-// [2^-40, 2^40]: for d2 = +0 or d2 in [2^-80, 2^80] the root is +0 or lies in
-// [2^-40, 2^40], inside both functions' domains; otherwise (rare) sqrtf and
-// the division
+// [2^-40, 2^39] (the uniform scenes' condition): for d2 = +0 or d2 in
+// [2^-80, 2^80] the root is +0 or lies in [2^-40, 2^40], inside both
+// functions' domains. A list entry of such a scene has d2 <= (2 h0)^2 <= 2^80
+// (its acceptance test), so only d2 in (0, 2^-80) (rare) takes sqrtf and the
+// division: one unsigned comparison of the bits (d2 is a sum of squares, +0 or
+// positive; for +0 the subtraction wraps).
 __device__ __forceinline__ float quotientH0(float d2, float h0, float y0)
 {
     float q = divBy(sqrtRanged(d2), h0, y0);
-    if (!(d2 == 0.0F || (d2 >= 0x1p-80F && d2 <= 0x1p80F)))
+    if (__float_as_uint(d2) - 1U < __float_as_uint(0x1p-80F) - 1U)
     {
         q = sqrtf(d2) / h0;
     }
@@ -1169,7 +1172,7 @@ __device__ __forceinline__ float listDensity(const Ctx& c, const SceneDev& sc, u
     if ((sc.flags & kSceneUniformHM) != 0U)
     {
         // every neighbour has h0 and m0: the same sum with one gather
-        // (the position) per pair; h0 lies in [2^-40, 2^40] (a condition of
+        // (the position) per pair; h0 lies in [2^-40, 2^39] (a condition of
         // the flag), so d / h0 goes through its reciprocal
         const float h0 = sc.h0;
         const float m0 = sc.m0;
@@ -1965,7 +1968,7 @@ __device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
 // Fluid pair sums of slot t over its neighbour list, in list order. With
 // kUniform every neighbour has h0 and m0 (two gathers per pair), otherwise
 // the mass and smoothing length are gathered too.
-// With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^40]) the quotient
+// With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^39]) the quotient
 // d / h0 is taken through the reciprocal of h0.
 template <bool kUniform, bool kSharedH = false, bool kLam = false>
 __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t lp, uint32_t cnt,
@@ -2151,7 +2154,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         PairAcc A {};
         if ((sc.flags & kSceneUniformHM) != 0U)
         {
-            // (the flag is set only for h0 in [2^-40, 2^40])
+            // (the flag is set only for h0 in [2^-40, 2^39])
             if (nuLam != 0.0F)
             {
                 forceList<true, true, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
