@@ -1649,14 +1649,17 @@ struct PairAcc
 // wdj = its mass, smoothing length and kWDCoef / h^4.
 // With kSharedH, yh = divRecip(hj) and hj lies in [2^-40, 2^40], so d / hj
 // is taken through the shared reciprocal (same bits as the division).
-template <bool kSharedH = false, bool kLam = false>
+template <bool kSharedH = false, bool kLam = false, bool kViscNoST = false>
 __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, float hj, float wdj, const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
                                           bool doST, float nuLam, PairAcc& A, float yh = 0.0F, bool use = true)
 {
     // use == false (the particle itself) computes and discards: the pair is
     // evaluated unconditionally so that the callers' gathers are not sunk
-    // into a branch (which serialises their latency)
+    // into a branch (which serialises their latency). kViscNoST: viscosity
+    // on and surface tension off, known at compile time (no branch on them).
+    const bool visc = kViscNoST ? true : doVisc;
+    const bool st = kViscNoST ? false : doST;
     const F4& o = fa;
     const float ox = q.x - o.x;
     const float oy = q.y - o.y;
@@ -1686,18 +1689,21 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
         const bool okz = oz == 0.0F || fabsf(oz) >= kDivLo;
         fast = okx && oky && okz && hi <= 0x1p39F && d2 >= 0x1p-80F;
     }
-    if (fast)
-    {
-        d = sqrtRanged(d2);
-        const float yd = divRecip(d);
-        dx = divBy(ox, d, yd);
-        dy = divBy(oy, d, yd);
-        dz = divBy(oz, d, yd);
-        qh = kSharedH ? divBy(d, hj, yh) : d / hj;
-    }
-    else
+    // the reciprocal path's results on every lane (the particle itself gets
+    // NaN quotients, which the selects below discard); lanes outside its
+    // domain (rare) recompute them with the divisions
+    d = sqrtRanged(d2);
+    const float yd = divRecip(d);
+    dx = divBy(ox, d, yd);
+    dy = divBy(oy, d, yd);
+    dz = divBy(oz, d, yd);
+    qh = kSharedH ? divBy(d, hj, yh) : d / hj;
+    if (!fast)
     {
         d = sqrtf(d2);
+        dx = 0.0F;
+        dy = 1.0F;
+        dz = 0.0F;
         if (d > 0.0F)
         {
             dx = ox / d;
@@ -1714,7 +1720,7 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
     A.apx = use ? px : A.apx;
     A.apy = use ? py : A.apy;
     A.apz = use ? pz : A.apz;
-    if (doVisc)
+    if (visc)
     {
         const float vx = vi.x - fb.x;
         const float vy = vi.y - fb.y;
@@ -1740,7 +1746,7 @@ __device__ __forceinline__ void forcePair(const F4& fa, const F4& fb, float mj, 
         A.amy = A.amy + sl * (vi.y - fb.y);
         A.amz = A.amz + sl * (vi.z - fb.z);
     }
-    if (doST && use)
+    if (st && use)
     {
         const float sw = mj * wendland(d, hj);
         A.asx = A.asx + sw * dx;
@@ -1970,7 +1976,7 @@ __device__ __forceinline__ F4 ownRecord(const Ctx& c, uint32_t t)
 // the mass and smoothing length are gathered too.
 // With kSharedH (uniform scenes whose h0 lies in [2^-40, 2^39]) the quotient
 // d / h0 is taken through the reciprocal of h0.
-template <bool kUniform, bool kSharedH = false, bool kLam = false>
+template <bool kUniform, bool kSharedH = false, bool kLam = false, bool kViscNoST = false>
 __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint32_t t, uint32_t lp, uint32_t cnt,
                                           const F4& q,
                                           const F4& vi, float rhoi, float prri, float nuNum, float epsH2, bool doVisc,
@@ -2011,7 +2017,7 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
 #pragma unroll
         for (uint32_t u = 0; u < B; ++u)
         {
-            forcePair<kSharedH, kLam>(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x,
+            forcePair<kSharedH, kLam, kViscNoST>(fa[u], fb[u], kUniform ? m0 : hm[u].y, kUniform ? h0 : hm[u].x,
                                 kUniform ? wd0 : hm[u].w, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A, y0,
                                 j[u] != t);
         }
@@ -2023,14 +2029,14 @@ __device__ __forceinline__ void forceList(const Ctx& c, const SceneDev& sc, uint
         {
             if (kUniform)
             {
-                forcePair<kSharedH, kLam>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
+                forcePair<kSharedH, kLam, kViscNoST>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), m0, h0, wd0, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
                                           nuLam, A,
                                     y0);
             }
             else
             {
                 const F4 h1 = recAt<F4>(c.shm, jj);
-                forcePair<false, kLam>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
+                forcePair<false, kLam, kViscNoST>(recAt<F4>(c.fa, jj), recAt<F4>(c.fb, jj), h1.y, h1.x, h1.w, q, vi, rhoi, prri, nuNum, epsH2, doVisc,
                                        doST, nuLam, A);
             }
         }
@@ -2159,6 +2165,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             {
                 forceList<true, true, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
             }
+            else if (doVisc && !doST)
+            {
+                forceList<true, true, false, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
+                                                   nuLam, A);
+            }
             else
             {
                 forceList<true, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
@@ -2169,6 +2180,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if (nuLam != 0.0F)
             {
                 forceList<false, false, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST, nuLam, A);
+            }
+            else if (doVisc && !doST)
+            {
+                forceList<false, false, false, true>(c, sc, t, pq, cnt, q, vi, rhoi, prri, nuNum, epsH2, doVisc, doST,
+                                                     nuLam, A);
             }
             else
             {
